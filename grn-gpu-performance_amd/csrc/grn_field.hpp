@@ -1,0 +1,359 @@
+// grn_field.hpp -- the on-chip field kernel (included by the per-width
+// translation units grn_field_w*.hip, which pick the instantiations).
+//
+// One workgroup simulates one model at a time; a persistent grid pulls models
+// from an atomic work queue sorted costliest-first.  Each model's whole field
+// stays on chip for all Euler steps:
+//   * every thread owns a vertical strip of CPT cells (column x, rows
+//     y0..y0+CPT-1) and keeps all G genes of those cells in registers as a
+//     compensated pair (hi, lo) of FP32 values;
+//   * every gene's plane is mirrored in LDS with a zero border, so a step
+//     reads east/west neighbours and the two strip-end neighbours from LDS and
+//     the in-strip north/south neighbours from registers;
+//   * Hill regulation is evaluated in exponent form (grn_program.h): one log2
+//     per gene, one FMA + min/max per link, one exp2 and one reciprocal per
+//     gene;
+//   * the per-step max |dc/dt| (the reference's convergence test) and the
+//     final fitness are reduced in the same kernel.
+// The gene count G is a template parameter: the kernel switches once per
+// model to the body compiled for that G, so gene loops are fully unrolled
+// with no per-gene conditions.
+//
+// Semantics follow the reference CPU simulator: Src/Simulator/
+// modelsimulator.cpp:205-280 (step), simulator.cpp:54-63 (stop rule),
+// simstate.cpp:135-158 (log error), Src/Search/evaluatormorphogens.cpp:48-76
+// (experiment loop, penalty, maxError cut).  Like the CPU path -- and unlike
+// the reference CUDA kernel -- every cell of the W x H grid is simulated and
+// scored.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "grn_kernels.h"
+#include "grn_program.h"
+
+namespace grn_field {
+
+// Per-cell log2 values, indexed by a uniform regulator index.  Sixteen lanes
+// keep the vector above the size at which the compiler turns a dynamic index
+// into a select chain, so each read is one indexed register move.
+typedef float lg_vec __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+constexpr float kMaxF = 3.402823466e38f;
+
+// Opaque values: the compiler cannot prove them loop-invariant, so loads and
+// address arithmetic derived from them stay where they are written instead of
+// being hoisted into long-lived registers.  opaque_zero() is uniform (it goes
+// through readfirstlane), so a pointer offset by it keeps its scalar loads.
+__device__ __forceinline__ int opaque_zero() {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return __builtin_amdgcn_readfirstlane(z);
+}
+__device__ __forceinline__ int opaque_v(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+struct Shared {
+  float *lds;      // planes (dynamic LDS)
+  float *wmax;     // per-wave max change
+  double *wsum;    // per-wave fitness partial sums
+};
+
+// Everything one model needs; G, CPT, WC compile-time.
+template <int WC, int G, int CPT, bool GUARD>
+struct Model {
+  const GrnProgram *P;
+  int W, H, plane;
+  float dt, min_conc;
+  bool clamp_nonzero_r;  // clamp only where R != 0 (needed when min_conc > 0)
+  bool active;
+  int x, y0, ubase;      // ubase: LDS index of the cell north-west of cell k=0
+  float nf[CPT];         // neighbour count per cell
+  float hi[G][CPT];
+  float lo[G][CPT];
+  Shared sh;
+
+  __device__ __forceinline__ int pc() const { return WC ? WC + 2 : W + 2; }
+
+  __device__ __forceinline__ void store_planes() {
+    if (!active) return;
+    float *pl = sh.lds + ubase + pc() + 1;
+#pragma unroll
+    for (int p = 0; p < G; ++p)
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) pl[p * plane + k * pc()] = hi[p][k];
+  }
+
+  // One Euler step of every owned cell, processed cell by cell down the
+  // strip.  With TRACK the workgroup-wide max |R| of the step is returned.
+  template <bool TRACK>
+  __device__ __forceinline__ float step() {
+    float prev[G];  // old values of the cell above (k-1)
+    float cmax = 0.f;
+
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const GrnProgram *Pq = P + opaque_zero();
+      lg_vec lg;  // lanes >= G are never read
+#pragma unroll
+      for (int g = 0; g < G; ++g) lg[g] = __builtin_fmaxf(hw_log2(hi[g][k]), -kMaxF);
+
+#pragma unroll
+      for (int p = 0; p < G; ++p) {
+        // The opaque index pins this gene's loads here; otherwise the
+        // scheduler lifts every gene's constants to the top of the step.
+        const GrnGene gp = Pq->gene[p + opaque_zero()];
+        const float c = hi[p][k];
+        float tor = gp.tor0, tand = gp.tand0;
+        for (int l = gp.link_begin; l < gp.link_begin + gp.n_or; ++l) {
+          const GrnLink L = Pq->link[l];
+          tor = __builtin_fminf(tor, __builtin_fmaf(L.a, lg[L.r], L.b));
+        }
+        for (int l = gp.link_begin + gp.n_or; l < gp.link_begin + gp.n_or + gp.n_and; ++l) {
+          const GrnLink L = Pq->link[l];
+          tand = __builtin_fmaxf(tand, __builtin_fmaf(L.a, lg[L.r], L.b));
+        }
+        const float h = hw_rcp(1.0f + hw_exp2(__builtin_fmaxf(tor, tand)));
+        float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
+
+        const float *pl = sh.lds + p * plane + ubase;
+        const float left = pl[(k + 1) * pc()];
+        const float right = pl[(k + 1) * pc() + 2];
+        const float up = (k == 0) ? pl[1] : prev[p];
+        const float down = (k == CPT - 1) ? pl[(CPT + 1) * pc() + 1] : hi[p][k + 1];
+        const float S = ((left + right) + up) + down;
+        float lap = __builtin_fmaf(-nf[k], c, S);
+        if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
+          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+          lap = use ? lap : 0.f;
+        }
+        R = __builtin_fmaf(gp.dif, lap, R);
+
+        // Compensated Euler update c += dt*R (Fast2Sum on the pair) with the
+        // minConc clamp of modelsimulator.cpp:263-275.
+        const float yv = __builtin_fmaf(dt, R, lo[p][k]);
+        const float sv = c + yv;
+        const float nl = yv - (sv - c);
+        const bool clamp = clamp_nonzero_r ? ((sv < min_conc) && (R != 0.f)) : (sv < min_conc);
+        hi[p][k] = clamp ? 0.f : sv;
+        lo[p][k] = clamp ? 0.f : nl;
+        if (TRACK) cmax = __builtin_fmaxf(cmax, __builtin_fabsf(R));
+        prev[p] = c;
+      }
+    }
+
+    if (TRACK) {
+      if (!active) cmax = 0.f;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) cmax = __builtin_fmaxf(cmax, __shfl_xor(cmax, off));
+    }
+    __syncthreads();  // every read of the old planes is done
+    store_planes();
+    if (TRACK && (threadIdx.x & 63) == 0) sh.wmax[threadIdx.x >> 6] = cmax;
+    __syncthreads();  // new planes visible
+    float chg = 0.f;
+    if (TRACK) {
+      const int nw = blockDim.x >> 6;
+      for (int w = 0; w < nw; ++w) chg = __builtin_fmaxf(chg, sh.wmax[w]);
+    }
+    return chg;
+  }
+
+  // Simulator::simulate (simulator.cpp:54-63): run while step < S and the
+  // last step's max change exceeds minConcChange.  A negative threshold can
+  // never stop the loop, so then only the final step needs the reduction.
+  __device__ __forceinline__ float run_steps(int S, float mcc, int *steps_done) {
+    float chg = 1.0f;
+    int s = 0;
+    if (mcc >= 0.f) {
+      while (s < S && chg > mcc) {
+        chg = step<true>();
+        ++s;
+      }
+    } else {
+      for (; s < S - 1; ++s) step<false>();
+      if (S > 0) {
+        chg = step<true>();
+        ++s;
+      }
+    }
+    *steps_done = s;
+    return chg;
+  }
+};
+
+// Simulates and scores one model (all experiments) -- the body of the
+// reference's simulateModelKernel (modelsimulatordevice.cu:216-251) with the
+// CPU semantics of EvaluatorMorphogens::evaluate.
+template <int WC, int G, int CPT, bool GUARD>
+__device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *__restrict__ progs,
+                                          const GrnDataset *__restrict__ datasets, int m, int pi,
+                                          Shared sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  Model<WC, G, CPT, GUARD> f;
+  const GrnProgram *P = progs + pi;
+  const GrnDataset *D = datasets + P->dataset;
+  f.P = P;
+  f.sh = sh;
+  f.W = WC ? WC : D->W;
+  f.H = D->H;
+  f.plane = f.pc() * (f.H + 2);
+  f.dt = P->dt;
+  f.min_conc = D->min_conc;
+  f.clamp_nonzero_r = D->min_conc > 0.f || !D->nonneg;
+  const int W = f.W, H = f.H, N = W * H;
+  const int n_active = W * (H / CPT);  // the launcher guarantees H % CPT == 0
+  f.active = tid < n_active;
+  f.x = f.active ? tid % W : 0;
+  f.y0 = f.active ? (tid / W) * CPT : 0;
+  f.ubase = f.y0 * f.pc() + f.x;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int y = f.y0 + k;
+    f.nf[k] = (float)((f.x > 0) + (f.x < W - 1) + (y > 0) + (y < H - 1));
+  }
+  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt, S = D->steps;
+  const float mcc = D->min_conc_change;
+  const double max_err = a.max_err[pi];
+  const int plane_floats = G * f.plane;
+
+  double err = 0.0, chg_acc = 0.0;
+  int e = 0, steps_total = 0;
+  while (e < n_exp && err <= max_err) {
+    // Initial state: input planes for genes < nIn, zero elsewhere
+    // (simulator.cpp:35-52); each double input is split into hi + lo.
+    {
+      const int cell = opaque_v(f.y0 * W + f.x);
+      const double *ip = D->init + (size_t)e * n_in * N + cell;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+          double v = 0.0;
+          if (g < n_in && f.active) v = ip[(size_t)g * N + k * W];
+          f.hi[g][k] = (float)v;
+          f.lo[g][k] = (float)(v - (double)f.hi[g][k]);
+        }
+    }
+    for (int i = tid; i < plane_floats; i += blockDim.x) sh.lds[i] = 0.f;
+    __syncthreads();
+    f.store_planes();
+    __syncthreads();
+
+    int sdone = 0;
+    const float chg = f.run_steps(S, mcc, &sdone);
+    steps_total += sdone;
+    chg_acc += (double)chg;
+
+    // Stage the final target planes in LDS as doubles (c = hi + lo), then
+    // score them in a plain loop.
+    double *stage = (double *)sh.lds;
+    if (f.active) {
+      const int cell = opaque_v(f.y0 * W + f.x);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g >= n_in && g < n_in + n_tgt)
+#pragma unroll
+          for (int k = 0; k < CPT; ++k)
+            stage[(g - n_in) * N + cell + k * W] = (double)f.hi[g][k] + (double)f.lo[g][k];
+      if (a.fields && e == a.field_exp) {
+        double *fo = a.fields + (size_t)m * a.field_stride + cell;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) fo[(size_t)g * N + k * W] = (double)f.hi[g][k] + (double)f.lo[g][k];
+      }
+    }
+    __syncthreads();
+
+    // REF: sum of log(1 + (|c - T| - thr)) over positive terms, / (W*H)
+    // (simstate.cpp:135-158).  MSE: mean of (c - T)^2 over cells and targets.
+    const bool mse = D->kind == 1;
+    const double thr = D->thr;
+    const double *tp = D->target + (size_t)e * n_tgt * N;
+    double acc = 0.0;
+    for (int i = tid; i < n_tgt * N; i += blockDim.x) {
+      const double d = stage[i] - tp[i];
+      if (mse) {
+        acc += d * d;
+      } else {
+        const double u = __builtin_fabs(d) - thr;
+        if (u > 0) acc += log(1.0 + u);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) sh.wsum[wave] = acc;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < nw; ++w) tot += sh.wsum[w];
+
+    double e_err;
+    if (mse) {
+      e_err = tot / ((double)N * n_tgt);
+    } else {
+      e_err = tot / N;
+      if (chg_acc > (double)mcc) e_err += chg_acc - (double)mcc;  // evaluatormorphogens.cpp:62-63
+    }
+    err += e_err / n_exp;
+    ++e;
+    __syncthreads();  // stage and wsum reads done before reuse
+  }
+
+  if (tid == 0) {
+    GrnModelOut o;
+    o.err = err;
+    o.change = chg_acc;
+    o.steps = steps_total;
+    o.exps = e;
+    o.done = 1;
+    o.pad = 0;
+    a.out[pi] = o;
+  }
+}
+
+template <int WC, int CPT, bool GUARD, int G, int GHI>
+__device__ __forceinline__ void dispatch_g(int g, const GrnLaunch &a, const GrnProgram *__restrict__ progs,
+                                           const GrnDataset *__restrict__ datasets, int m, int pi,
+                                           Shared sh) {
+  if constexpr (G <= GHI) {
+    if (g == G)
+      run_model<WC, G, CPT, GUARD>(a, progs, datasets, m, pi, sh);
+    else
+      dispatch_g<WC, CPT, GUARD, G + 1, GHI>(g, a, progs, datasets, m, pi, sh);
+  }
+}
+
+// Persistent kernel: models with GLO <= G <= GHI genes of one grid width.
+template <int WC, int CPT, int NTMAX, int GLO, int GHI, bool GUARD>
+__global__ __launch_bounds__(NTMAX) void field_kernel(GrnLaunch a, const GrnProgram *__restrict__ progs,
+                                                      const GrnDataset *__restrict__ datasets,
+                                                      const int *__restrict__ order) {
+  extern __shared__ float lds[];
+  __shared__ int s_model;
+  __shared__ float s_wmax[NTMAX / 64];
+  __shared__ double s_wsum[NTMAX / 64];
+  Shared sh{lds, s_wmax, s_wsum};
+  for (;;) {
+    if (threadIdx.x == 0) s_model = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int m = __builtin_amdgcn_readfirstlane(s_model);
+    __syncthreads();
+    if (m >= a.n) break;
+    const int pi = __builtin_amdgcn_readfirstlane(order[m]);
+    const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
+    dispatch_g<WC, CPT, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
+    __syncthreads();
+  }
+}
+
+}  // namespace grn_field
+
+#define GRN_FIELD_ENTRY(WC, CPT, NTMAX, GLO, GHI, GUARD) \
+  {WC, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::field_kernel<WC, CPT, NTMAX, GLO, GHI, GUARD>}

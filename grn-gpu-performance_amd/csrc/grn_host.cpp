@@ -1,0 +1,541 @@
+// grn_host.cpp -- the C ABI of libgrn_hip.so: device context, experiment
+// upload, model compilation, kernel dispatch and result collection.
+//
+// Host-side mirror of Src/GPUSimulator (CEvaluatorMorphogens,
+// CModelSimulator): one context per device and caller thread, one HIP stream
+// per context, experiment planes uploaded once, models compiled on the host
+// and evaluated by the fused on-chip kernel.  Batches are grouped by kernel
+// class (grid width, gene range), ordered costliest-first and run as one
+// persistent launch per class, synchronizing once per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "grn.h"
+#include "grn_kernels.h"
+#include "grn_model.hpp"
+#include "grn_program.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(GRN_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int reserve(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    if (hipMalloc(&p, want) != hipSuccess) return fail(GRN_E_NOMEM, "hipMalloc(%zu) failed", want);
+    cap = want;
+    return 0;
+  }
+};
+
+struct HostDataset {
+  grn_sim_params params;
+  int n_exp;
+  bool nonneg;
+  void *d_init = nullptr;
+  void *d_target = nullptr;
+};
+
+struct KernelChoice {
+  const GrnKernelEntry *k = nullptr;
+  int block = 0;
+  size_t lds = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0;
+  size_t max_lds = 0;
+  std::vector<HostDataset> datasets;
+  DevBuf d_datasets, d_progs, d_order, d_maxerr, d_out, d_counter, d_fields;
+  std::vector<const GrnKernelEntry *> kernels;
+  std::map<const void *, size_t> lds_set;  // dynamic-LDS attribute already applied
+  grn_stats stats{};
+
+  ~Ctx() {
+    for (HostDataset &d : datasets) {
+      if (d.d_init) (void)hipFree(d.d_init);
+      if (d.d_target) (void)hipFree(d.d_target);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
+  const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w48,
+                                              grn_field_table_w64, grn_field_table_wany};
+  for (auto fn : tables) {
+    int n = 0;
+    const GrnKernelEntry *t = fn(&n);
+    for (int i = 0; i < n; ++i) out->push_back(&t[i]);
+  }
+}
+
+// Picks the on-chip kernel for a grid of W x H with G genes, or returns an
+// error when no instantiation can hold it.
+int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, KernelChoice *out) {
+  const KernelChoice none;
+  *out = none;
+  for (int pass = 0; pass < 2; ++pass) {  // exact-width kernels first, then any-width
+    for (const GrnKernelEntry *k : c.kernels) {
+      if (pass == 0 ? k->wc != W : k->wc != 0) continue;
+      if (G < k->glo || G > k->ghi || k->guard != guard) continue;
+      if (H % k->cpt) continue;
+      const int active = W * (H / k->cpt);
+      const int block = (active + 63) / 64 * 64;
+      if (block > k->ntmax) continue;
+      const size_t planes = (size_t)G * (W + 2) * (H + 2) * sizeof(float);
+      const size_t stage = (size_t)n_tgt * W * H * sizeof(double);
+      const size_t lds = std::max(planes, stage);
+      if (lds > c.max_lds) continue;
+      out->k = k;
+      out->block = block;
+      out->lds = lds;
+      return 0;
+    }
+  }
+  return fail(GRN_E_CAPACITY,
+              "no on-chip kernel for a %dx%d grid with %d genes (LDS %zu bytes available)", W, H, G,
+              c.max_lds);
+}
+
+Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+int to_model(const grn_model *gm, grn::Model *m) {
+  if (!gm || gm->n_genes < 0 || gm->n_links < 0 || (gm->n_genes && !gm->genes) ||
+      (gm->n_links && !gm->links))
+    return fail(GRN_E_INVALID, "invalid grn_model");
+  m->genes.resize(gm->n_genes);
+  for (int i = 0; i < gm->n_genes; ++i) {
+    const grn_gene &g = gm->genes[i];
+    m->genes[i] = grn::Gene{g.label, g.init, g.lim, g.deg, g.dif};
+  }
+  m->links.resize(gm->n_links);
+  for (int i = 0; i < gm->n_links; ++i) {
+    const grn_link &l = gm->links[i];
+    m->links[i] = grn::Link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and != 0};
+  }
+  return 0;
+}
+
+struct Item {
+  int dataset;
+  GrnProgram prog;
+  grn::ProgramInfo info;
+  KernelChoice kc;
+  double cost;
+  double max_err;
+};
+
+// Compiles, uploads, launches and collects one batch.  fields (optional)
+// receives each model's final field of experiment field_exp.
+int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
+              const double *max_errors, const double *dt_override, grn_result *out,
+              double *fields_host, long field_stride, int field_exp, int *n_products_out) {
+  const auto t_start = std::chrono::steady_clock::now();
+  if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
+  std::vector<Item> items(n);
+  for (int i = 0; i < n; ++i) {
+    const int ds = dataset_ids ? dataset_ids[i] : 0;
+    if (ds < 0 || ds >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "model %d: unknown dataset %d", i, ds);
+    const HostDataset &D = c->datasets[ds];
+    grn::Model m;
+    int rc = to_model(&models[i], &m);
+    if (rc) return rc;
+    const double dt = (dt_override && dt_override[i] > 0) ? dt_override[i] : D.params.dt;
+    std::string err;
+    const int n_morph = D.params.n_input_morphogens + D.params.n_target_morphogens;
+    rc = grn::compile_model(m, n_morph, ds, (float)dt, &items[i].prog, &items[i].info, &err);
+    if (rc) return fail(rc, "model %d: %s", i, err.c_str());
+    const int W = D.params.dim_x, H = D.params.dim_y;
+    rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, &items[i].kc);
+    if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
+    items[i].dataset = ds;
+    items[i].cost = (double)W * H * D.params.num_sim_steps * D.n_exp * items[i].info.flops_per_cell_step;
+    items[i].max_err = max_errors ? max_errors[i] : std::numeric_limits<double>::infinity();
+  }
+
+  // Group by kernel, costliest first inside each group (longest-processing-
+  // time order for the persistent work queue).
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    if (items[a].kc.k != items[b].kc.k) return items[a].kc.k < items[b].kc.k;
+    return items[a].cost > items[b].cost;
+  });
+
+  std::vector<GrnProgram> progs(n);
+  std::vector<double> maxerr(n);
+  for (int i = 0; i < n; ++i) {
+    progs[i] = items[i].prog;
+    maxerr[i] = items[i].max_err;
+  }
+  int rc;
+  if ((rc = c->d_progs.reserve(sizeof(GrnProgram) * std::max(n, 1)))) return rc;
+  if ((rc = c->d_order.reserve(sizeof(int) * std::max(n, 1)))) return rc;
+  if ((rc = c->d_maxerr.reserve(sizeof(double) * std::max(n, 1)))) return rc;
+  if ((rc = c->d_out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
+  if ((rc = c->d_counter.reserve(sizeof(int) * 64))) return rc;
+  if (fields_host) {
+    if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
+    for (int i = 1; i < n; ++i)
+      if (items[i].kc.k != items[0].kc.k) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
+  }
+  hipStream_t st = c->stream;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(c->d_progs.p, progs.data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_order.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_maxerr.p, maxerr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(c->d_out.p, 0, sizeof(GrnModelOut) * n, st));
+  }
+
+  // One persistent launch per kernel class.
+  std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
+  for (int i = 0; i < n;) {
+    int j = i;
+    while (j < n && items[idx[j]].kc.k == items[idx[i]].kc.k) ++j;
+    groups.push_back({i, j});
+    i = j;
+  }
+  if ((int)groups.size() > 64) return fail(GRN_E_CAPACITY, "too many kernel classes in one batch");
+  HIP_TRY(hipMemsetAsync(c->d_counter.p, 0, sizeof(int) * 64, st));
+  HIP_TRY(hipEventRecord(c->ev0, st));
+  int launches = 0;
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const int b = groups[gi].first, e = groups[gi].second;
+    // the class's block size and LDS: the largest over its members
+    KernelChoice kc = items[idx[b]].kc;
+    for (int j = b; j < e; ++j) {
+      kc.block = std::max(kc.block, items[idx[j]].kc.block);
+      kc.lds = std::max(kc.lds, items[idx[j]].kc.lds);
+    }
+    auto it = c->lds_set.find(kc.k->fn);
+    if (it == c->lds_set.end() || it->second < kc.lds) {
+      HIP_TRY(hipFuncSetAttribute(kc.k->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kc.lds));
+      c->lds_set[kc.k->fn] = kc.lds;
+    }
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
+    if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
+    const int grid = std::min(e - b, per_cu * c->n_cu);
+    GrnLaunch L{};
+    L.progs = (const GrnProgram *)c->d_progs.p;
+    L.order = (const int *)c->d_order.p + b;
+    L.n = e - b;
+    L.counter = (int *)c->d_counter.p + gi;
+    L.datasets = (const GrnDataset *)c->d_datasets.p;
+    L.max_err = (const double *)c->d_maxerr.p;
+    L.out = (GrnModelOut *)c->d_out.p;
+    L.fields = fields_host ? (double *)c->d_fields.p : nullptr;
+    L.field_stride = field_stride;
+    L.field_exp = field_exp;
+    const GrnProgram *pp = L.progs;
+    const GrnDataset *dp = L.datasets;
+    const int *op = L.order;
+    void *args[] = {&L, &pp, &dp, &op};
+    HIP_TRY(hipLaunchKernel(kc.k->fn, dim3(grid), dim3(kc.block), args, kc.lds, st));
+    ++launches;
+  }
+  HIP_TRY(hipEventRecord(c->ev1, st));
+
+  std::vector<GrnModelOut> res(n);
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(res.data(), c->d_out.p, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
+  if (fields_host && n > 0) {
+    // d_fields rows are indexed by queue position; return them in input order.
+    std::vector<double> tmp((size_t)field_stride * n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), c->d_fields.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // The kernel writes a model's field at its queue position inside its
+    // launch; map the rows back to input order.
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      for (int j = groups[gi].first; j < groups[gi].second; ++j) {
+        const int row = j - groups[gi].first;
+        std::memcpy(fields_host + (size_t)field_stride * idx[j], tmp.data() + (size_t)field_stride * row,
+                    sizeof(double) * field_stride);
+      }
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+
+  double cell_steps = 0, flops = 0;
+  for (int i = 0; i < n; ++i) {
+    const HostDataset &D = c->datasets[items[i].dataset];
+    const GrnModelOut &o = res[i];
+    if (!o.done) return fail(GRN_E_RESULT, "model %d: the kernel wrote no result", i);
+    grn_result &r = out[i];
+    r.raw_error = o.err;
+    r.change = o.change;
+    r.steps = o.steps;
+    r.experiments = o.exps;
+    r.n_products = items[i].prog.G;
+    r.reserved = 0;
+    // CEvaluatorMorphogens::evaluate (cevaluatormorphogens.cpp:239-242)
+    if (o.err > items[i].max_err)
+      r.fitness = 100;
+    else if (D.params.error_precision > 0)
+      r.fitness = grn_ceil_s(o.err, D.params.error_precision);
+    else
+      r.fitness = o.err;
+    const double cs = (double)D.params.dim_x * D.params.dim_y * o.steps;
+    cell_steps += cs;
+    flops += cs * items[i].info.flops_per_cell_step;
+    if (n_products_out) n_products_out[i] = items[i].prog.G;
+  }
+  c->stats.kernel_ms = ms;
+  c->stats.cell_steps = cell_steps;
+  c->stats.alg_flops = flops;
+  c->stats.n_launches = launches;
+  c->stats.n_models = n;
+  c->stats.host_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grn_abi_version(void) { return GRN_ABI_VERSION; }
+
+const char *grn_last_error(void) { return g_last_error.c_str(); }
+
+double grn_ceil_s(double n, int d) {
+  // MathAlgo::ceilS (Src/Common/mathalgo.h:55-62)
+  if (n == 0.0) return 0.0;
+  const double factor = std::pow(10.0, d - std::ceil(std::log10(std::fabs(n))));
+  return std::ceil(n * factor) / factor;
+}
+
+int grn_device_count(int *n) {
+  if (!n) return fail(GRN_E_INVALID, "null output");
+  int k = 0;
+  HIP_TRY(hipGetDeviceCount(&k));
+  *n = k;
+  return 0;
+}
+
+int grn_create(int device, void **out) {
+  if (!out) return fail(GRN_E_INVALID, "null output");
+  *out = nullptr;
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(GRN_E_INVALID, "device %d of %d", device, count);
+  HIP_TRY(hipSetDevice(device));
+  std::unique_ptr<Ctx> c(new Ctx);
+  c->device = device;
+  HIP_TRY(hipStreamCreate(&c->stream));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+  int lds = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  // Static LDS of the kernels (queue slot, per-wave partials) stays below 1 KB.
+  c->max_lds = lds > 1024 ? (size_t)lds - 1024 : 0;
+  collect_kernels(&c->kernels);
+  *out = c.release();
+  return 0;
+}
+
+void grn_destroy(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete c;
+}
+
+int grn_add_dataset(void *ctx, const grn_sim_params *p, int n_exp, const double *init,
+                    const double *target, int *dataset_id) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !p || !dataset_id) return fail(GRN_E_INVALID, "null argument");
+  if (p->dim_x < 1 || p->dim_y < 1 || p->n_input_morphogens < 0 || p->n_target_morphogens < 0 ||
+      n_exp < 1 || p->num_sim_steps < 0 || p->fitness_kind < 0 || p->fitness_kind > 1)
+    return fail(GRN_E_INVALID, "invalid simulation parameters");
+  if (p->n_target_morphogens == 0 && p->fitness_kind == GRN_FITNESS_MSE)
+    return fail(GRN_E_INVALID, "MSE fitness needs at least one target plane");
+  const size_t N = (size_t)p->dim_x * p->dim_y;
+  const size_t n_init = (size_t)n_exp * p->n_input_morphogens * N;
+  const size_t n_tgt = (size_t)n_exp * p->n_target_morphogens * N;
+  if ((n_init && !init) || (n_tgt && !target)) return fail(GRN_E_INVALID, "missing pattern data");
+  HIP_TRY(hipSetDevice(c->device));
+  HostDataset D;
+  D.params = *p;
+  D.n_exp = n_exp;
+  bool nonneg = p->min_conc >= 0;
+  for (size_t i = 0; i < n_init && nonneg; ++i) nonneg = init[i] >= 0;
+  D.nonneg = nonneg;
+  HIP_TRY(hipMalloc(&D.d_init, std::max(n_init, (size_t)1) * sizeof(double)));
+  HIP_TRY(hipMalloc(&D.d_target, std::max(n_tgt, (size_t)1) * sizeof(double)));
+  if (n_init) HIP_TRY(hipMemcpyAsync(D.d_init, init, n_init * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (n_tgt) HIP_TRY(hipMemcpyAsync(D.d_target, target, n_tgt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  c->datasets.push_back(D);
+
+  std::vector<GrnDataset> dev(c->datasets.size());
+  for (size_t i = 0; i < c->datasets.size(); ++i) {
+    const HostDataset &h = c->datasets[i];
+    GrnDataset &g = dev[i];
+    std::memset(&g, 0, sizeof g);
+    g.W = h.params.dim_x;
+    g.H = h.params.dim_y;
+    g.n_in = h.params.n_input_morphogens;
+    g.n_tgt = h.params.n_target_morphogens;
+    g.n_exp = h.n_exp;
+    g.steps = h.params.num_sim_steps;
+    g.kind = h.params.fitness_kind;
+    g.nonneg = h.nonneg ? 1 : 0;
+    g.min_conc = (float)h.params.min_conc;
+    g.min_conc_change = (float)h.params.min_conc_change;
+    g.thr = h.params.dist_error_threshold;
+    g.init = (const double *)h.d_init;
+    g.target = (const double *)h.d_target;
+  }
+  int rc = c->d_datasets.reserve(sizeof(GrnDataset) * dev.size());
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->d_datasets.p, dev.data(), sizeof(GrnDataset) * dev.size(), hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *dataset_id = (int)c->datasets.size() - 1;
+  return 0;
+}
+
+int grn_evaluate(void *ctx, int dataset_id, const grn_model *model, double max_error, grn_result *out) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  HIP_TRY(hipSetDevice(c->device));
+  return run_batch(c, 1, &dataset_id, model, &max_error, nullptr, out, nullptr, 0, 0, nullptr);
+}
+
+int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
+                       const double *max_errors, const double *dt_override, int path, grn_result *out) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  if (path == GRN_PATH_TILED) return fail(GRN_E_CAPACITY, "the halo-tiled path is not built");
+  HIP_TRY(hipSetDevice(c->device));
+  return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, out, nullptr, 0, 0, nullptr);
+}
+
+int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, double dt_override, int experiment,
+                        int path, double *fields, long capacity, int *n_products) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !fields) return fail(GRN_E_INVALID, "null argument");
+  if (path == GRN_PATH_TILED) return fail(GRN_E_CAPACITY, "the halo-tiled path is not built");
+  if (dataset_id < 0 || dataset_id >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "unknown dataset");
+  const grn_sim_params &p = c->datasets[dataset_id].params;
+  if (experiment < 0 || experiment >= c->datasets[dataset_id].n_exp) return fail(GRN_E_INVALID, "bad experiment");
+  const long stride = (long)GRN_PMAX_GENES * p.dim_x * p.dim_y;
+  HIP_TRY(hipSetDevice(c->device));
+  std::vector<double> buf(stride);
+  grn_result r;
+  int g = 0;
+  const double dt = dt_override;
+  const double inf = std::numeric_limits<double>::infinity();
+  int rc = run_batch(c, 1, &dataset_id, model, &inf, &dt, &r, buf.data(), stride, experiment, &g);
+  if (rc) return rc;
+  const long need = (long)g * p.dim_x * p.dim_y;
+  if (capacity < need) return fail(GRN_E_INVALID, "fields buffer holds %ld of %ld values", capacity, need);
+  std::memcpy(fields, buf.data(), sizeof(double) * need);
+  if (n_products) *n_products = g;
+  return 0;
+}
+
+int grn_get_stats(void *ctx, grn_stats *out) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !out) return fail(GRN_E_INVALID, "null argument");
+  *out = c->stats;
+  return 0;
+}
+
+int grn_model_parse(const char *text, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                    int link_cap, int *n_links) {
+  if (!text || !n_genes || !n_links) return fail(GRN_E_INVALID, "null argument");
+  grn::Model m;
+  std::string err;
+  if (!grn::Model::parse(text, &m, &err)) return fail(GRN_E_MODEL, "%s", err.c_str());
+  *n_genes = (int)m.genes.size();
+  *n_links = (int)m.links.size();
+  if ((int)m.genes.size() > gene_cap || (int)m.links.size() > link_cap)
+    return fail(GRN_E_CAPACITY, "model has %zu genes and %zu links", m.genes.size(), m.links.size());
+  for (size_t i = 0; i < m.genes.size(); ++i) {
+    const grn::Gene &g = m.genes[i];
+    genes[i] = grn_gene{g.label, g.init, g.lim, g.deg, g.dif};
+  }
+  for (size_t i = 0; i < m.links.size(); ++i) {
+    const grn::Link &l = m.links[i];
+    links[i] = grn_link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and ? 1 : 0};
+  }
+  return 0;
+}
+
+int grn_model_format(const grn_model *model, int precision, char *buf, long cap, long *len) {
+  grn::Model m;
+  int rc = to_model(model, &m);
+  if (rc) return rc;
+  const std::string s = m.to_string(precision > 0 ? precision : 6);
+  if (len) *len = (long)s.size();
+  if (!buf || cap < (long)s.size() + 1) return fail(GRN_E_CAPACITY, "buffer holds %ld of %zu bytes", cap, s.size() + 1);
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return 0;
+}
+
+int grn_model_info(const grn_model *model, int n_morphogens, int *genes_in_use, int *hill_links,
+                   double *flops_per_cell_step) {
+  grn::Model m;
+  int rc = to_model(model, &m);
+  if (rc) return rc;
+  GrnProgram prog;
+  grn::ProgramInfo info;
+  std::string err;
+  rc = grn::compile_model(m, n_morphogens, 0, 0.f, &prog, &info, &err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  if (genes_in_use) *genes_in_use = info.G;
+  if (hill_links) *hill_links = info.L;
+  if (flops_per_cell_step) *flops_per_cell_step = info.flops_per_cell_step;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,21 @@
+// grn_kernels.h -- the kernel instantiations the launcher can pick from.
+#ifndef GRN_KERNELS_H
+#define GRN_KERNELS_H
+
+struct GrnKernelEntry {
+  int wc;     // compile-time grid width, 0 = any width
+  int cpt;    // cells per thread (vertical strip length)
+  int ntmax;  // workgroup size bound
+  int glo;    // smallest gene count handled
+  int ghi;    // largest gene count handled
+  bool guard; // applies the diffusion guard (needed for fields that can go negative)
+  const void *fn;
+};
+
+// Instantiations of the on-chip field kernel, one table per translation unit.
+extern "C" const GrnKernelEntry *grn_field_table_w32(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
+
+#endif

@@ -1,0 +1,91 @@
+// grn_program.h -- device-side layout of a compiled GRN model and of an
+// uploaded experiment set.  Shared by the host compiler (grn_compile.cpp),
+// the host launcher (grn_host.cpp) and the kernels (grn_kernels.hip).
+//
+// A model is compiled on the host into a GrnProgram: the in-use genes in
+// ascending label order (Src/Simulator/modelsimulator.cpp:71-80), their
+// lim/deg/dif constants, and for every gene whose links include an activator
+// (modelsimulator.cpp:120-138) its OR links followed by its AND links, each
+// in model order (modelsimulator.cpp:156-192).
+//
+// Hill terms are carried in exponent form.  For a link with regulator
+// concentration x, dissociation constant K and exponent n = |hillCoef|:
+//   activation  x^n/(x^n+K^n) = 1/(1+2^t),  t = -n*log2(x) + n*log2(K)
+//   repression  K^n/(x^n+K^n) = 1/(1+2^t),  t = +n*log2(x) - n*log2(K)
+// so a link is (r, a, b) with t = a*log2(x_r) + b.  Because 1/(1+2^t) falls
+// monotonically in t, the reference's fold h = min(max_OR H, min_AND H)
+// (simopor.cpp, simopand.cpp) becomes one exponential per gene:
+//   h = 1/(1 + 2^T),  T = max(min_OR t, max_AND t)
+// with an empty OR or AND side dropped.
+#ifndef GRN_PROGRAM_H
+#define GRN_PROGRAM_H
+
+#define GRN_PMAX_GENES 16
+#define GRN_PMAX_LINKS 96
+
+struct GrnLink {
+  int r;    // regulator gene index (0..G-1)
+  float a;  // t = a*log2(x) + b
+  float b;
+  int pad;
+};
+
+// Per-gene constants.  tor0 / tand0 seed the OR-min and AND-max folds so
+// that T = max(tor, tand) needs no branches:
+//   OR links present:  tor0 = +inf (min identity), else -inf (side dropped)
+//   Hill links present: tand0 = -inf (max identity)
+//   no activator among the gene's links (modelsimulator.cpp:120-138: h = 0):
+//     no links stored, tor0 = -inf, tand0 = +inf, so T = +inf and h = 0.
+struct GrnGene {
+  float lim, deg, dif;
+  float tor0, tand0;
+  int link_begin, n_or, n_and;
+};
+
+struct GrnProgram {
+  int G;              // genes in use (every gene has an LDS plane)
+  int n_links;        // links stored below
+  int dataset;        // experiment set
+  float dt;           // Euler time step
+  int pad[4];
+  GrnGene gene[GRN_PMAX_GENES];
+  GrnLink link[GRN_PMAX_LINKS];
+};
+
+struct GrnDataset {
+  int W, H;
+  int n_in, n_tgt;
+  int n_exp, steps;
+  int kind;       // 0 = REF, 1 = MSE
+  int nonneg;     // inputs >= 0 and min_conc >= 0: the diffusion guard is moot
+  float min_conc;
+  float min_conc_change;
+  double thr;     // distErrorThreshold
+  const double *init;    // [n_exp][n_in][H][W]
+  const double *target;  // [n_exp][n_tgt][H][W]
+};
+
+struct GrnModelOut {
+  double err;     // accumulated error (before sentinel / rounding)
+  double change;  // accumulated last-step max change
+  int steps;      // Euler steps executed, summed over experiments
+  int exps;       // experiments simulated
+  int done;       // 1 once written
+  int pad;
+};
+
+struct GrnLaunch {
+  const GrnProgram *progs;  // [n]
+  const int *order;         // work order: program indices, costliest first
+  int n;
+  int *counter;             // work-queue head, zero at launch
+  const GrnDataset *datasets;
+  const double *max_err;    // [n]
+  GrnModelOut *out;         // [n]
+  double *fields;           // optional: [n][G][H][W] final field
+  long field_stride;        // elements per model in fields
+  int field_exp;            // experiment whose final field is written
+  int pad;
+};
+
+#endif

@@ -1,0 +1,292 @@
+"""Python binding of libgrn_hip.so (the C ABI declared in include/grn.h).
+
+Mirrors the reference's GPU evaluator interface: a Context is
+CEvaluatorMorphogens (Src/GPUSimulator/cevaluatormorphogens.h:21-62) -- device
+init plus experiment upload -- and Context.evaluate(model, max_error) is
+CEvaluatorMorphogens::evaluate (cevaluatormorphogens.cpp:233-245).  Models
+are the reference's Model (genes = ModelProd, links = ModelLink) as numpy
+structured arrays.
+
+The library must exist (build it with ``python -c "import
+__graft_entry__ as g; g.build()"`` or ``make -C grn-gpu-performance_amd``);
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgrn_hip.so")
+
+FITNESS_REF = 0
+FITNESS_MSE = 1
+PATH_AUTO, PATH_LDS, PATH_TILED = 0, 1, 2
+MAX_GENES = 16
+MAX_LINKS = 96
+
+GENE_DTYPE = np.dtype({"names": ["label", "init", "lim", "deg", "dif"],
+                       "formats": [np.int32, np.float64, np.float64, np.float64, np.float64],
+                       "offsets": [0, 8, 16, 24, 32], "itemsize": 40})
+LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_coef", "is_and"],
+                       "formats": [np.int32, np.int32, np.float64, np.float64, np.int32],
+                       "offsets": [0, 4, 8, 16, 24], "itemsize": 32})
+
+EXPORTED = [
+    "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
+    "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
+    "grn_get_stats", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
+]
+
+
+class GrnError(RuntimeError):
+    """A grn_* call returned a negative grn_status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"grn error {code}: {msg}")
+        self.code = code
+
+
+class _SimParams(ctypes.Structure):
+    _fields_ = [("dim_x", ctypes.c_int), ("dim_y", ctypes.c_int),
+                ("n_input_morphogens", ctypes.c_int), ("n_target_morphogens", ctypes.c_int),
+                ("num_sim_steps", ctypes.c_int), ("dt", ctypes.c_double),
+                ("min_conc", ctypes.c_double), ("min_conc_change", ctypes.c_double),
+                ("dist_error_threshold", ctypes.c_double), ("error_precision", ctypes.c_int),
+                ("fitness_kind", ctypes.c_int)]
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [("n_genes", ctypes.c_int), ("genes", ctypes.c_void_p),
+                ("n_links", ctypes.c_int), ("links", ctypes.c_void_p)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("fitness", ctypes.c_double), ("raw_error", ctypes.c_double),
+                ("change", ctypes.c_double), ("n_products", ctypes.c_int),
+                ("steps", ctypes.c_int), ("experiments", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
+                ("cell_steps", ctypes.c_double), ("alg_flops", ctypes.c_double),
+                ("n_launches", ctypes.c_int), ("n_models", ctypes.c_int)]
+
+
+RESULT_DTYPE = np.dtype({"names": ["fitness", "raw_error", "change", "n_products", "steps",
+                                   "experiments", "reserved"],
+                         "formats": [np.float64, np.float64, np.float64, np.int32, np.int32,
+                                     np.int32, np.int32],
+                         "offsets": [0, 8, 16, 24, 28, 32, 36], "itemsize": 40})
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libgrn_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build the HIP library first")
+    lib = ctypes.CDLL(path)
+    vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    lib.grn_abi_version.restype = ctypes.c_int
+    lib.grn_last_error.restype = ctypes.c_char_p
+    lib.grn_device_count.argtypes = [ip]
+    lib.grn_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.grn_destroy.argtypes = [vp]
+    lib.grn_destroy.restype = None
+    lib.grn_add_dataset.argtypes = [vp, ctypes.POINTER(_SimParams), ctypes.c_int, vp, vp, ip]
+    lib.grn_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
+                                 ctypes.POINTER(_Result)]
+    lib.grn_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp]
+    lib.grn_simulate_fields.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, vp, ctypes.c_long, ip]
+    lib.grn_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
+    lib.grn_ceil_s.argtypes = [ctypes.c_double, ctypes.c_int]
+    lib.grn_ceil_s.restype = ctypes.c_double
+    lib.grn_model_parse.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip]
+    lib.grn_model_format.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, ctypes.c_char_p,
+                                     ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    lib.grn_model_info.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, ip, ip, dp]
+    for name in EXPORTED:
+        if name not in ("grn_last_error", "grn_destroy", "grn_ceil_s", "grn_abi_version"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise GrnError(rc, load_library().grn_last_error().decode(errors="replace"))
+
+
+# ---- models -------------------------------------------------------------------
+
+@dataclass
+class Model:
+    """Model (Src/Model/model.h): genes = ModelProd rows, links = ModelLink rows."""
+    genes: np.ndarray = field(default_factory=lambda: np.zeros(0, GENE_DTYPE))
+    links: np.ndarray = field(default_factory=lambda: np.zeros(0, LINK_DTYPE))
+
+    def __post_init__(self):
+        self.genes = np.ascontiguousarray(self.genes, dtype=GENE_DTYPE)
+        self.links = np.ascontiguousarray(self.links, dtype=LINK_DTYPE)
+
+    def _c(self) -> _Model:
+        return _Model(len(self.genes), self.genes.ctypes.data if len(self.genes) else None,
+                      len(self.links), self.links.ctypes.data if len(self.links) else None)
+
+    @staticmethod
+    def from_text(text: str) -> "Model":
+        """Model::operator>> (Src/Model/model.cpp:659-706)."""
+        lib = load_library()
+        g = np.zeros(256, GENE_DTYPE)
+        l = np.zeros(1024, LINK_DTYPE)
+        ng, nl = ctypes.c_int(), ctypes.c_int()
+        _check(lib.grn_model_parse(text.encode(), g.ctypes.data, len(g), ctypes.byref(ng),
+                                   l.ctypes.data, len(l), ctypes.byref(nl)))
+        return Model(g[:ng.value].copy(), l[:nl.value].copy())
+
+    def to_text(self, precision: int = 6) -> str:
+        """Model::operator<< (Src/Model/model.cpp:627-657)."""
+        lib = load_library()
+        cm = self._c()
+        n = ctypes.c_long()
+        lib.grn_model_format(ctypes.byref(cm), precision, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(lib.grn_model_format(ctypes.byref(cm), precision, buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def info(self, n_morphogens: int):
+        """(genes in use, Hill links, algorithmic FLOPs per cell-step)."""
+        lib = load_library()
+        cm = self._c()
+        g, l, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib.grn_model_info(ctypes.byref(cm), n_morphogens, ctypes.byref(g), ctypes.byref(l),
+                                  ctypes.byref(f)))
+        return g.value, l.value, f.value
+
+
+def ceil_s(n: float, digits: int) -> float:
+    """MathAlgo::ceilS (Src/Common/mathalgo.h:55-62)."""
+    return load_library().grn_ceil_s(n, digits)
+
+
+@dataclass
+class SimParams:
+    """The SimParams fields of the path (Src/Simulator/simparams.h:25-43)."""
+    dim_x: int
+    dim_y: int
+    n_input_morphogens: int
+    n_target_morphogens: int
+    num_sim_steps: int
+    dt: float
+    min_conc: float = 0.0
+    min_conc_change: float = -1.0
+    dist_error_threshold: float = 0.0
+    error_precision: int = 0
+    fitness_kind: int = FITNESS_MSE
+
+    def _c(self) -> _SimParams:
+        return _SimParams(self.dim_x, self.dim_y, self.n_input_morphogens, self.n_target_morphogens,
+                          self.num_sim_steps, self.dt, self.min_conc, self.min_conc_change,
+                          self.dist_error_threshold, self.error_precision, self.fitness_kind)
+
+
+class Context:
+    """One device context: stream, uploaded experiment sets, device buffers."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.grn_create(device, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        self.params: list[SimParams] = []
+
+    def close(self):
+        if self._h:
+            self._lib.grn_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_dataset(self, params: SimParams, init: np.ndarray, target: np.ndarray) -> int:
+        """init [n_exp][n_in][H][W], target [n_exp][n_tgt][H][W] (float64)."""
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        n_exp = init.shape[0] if init.ndim == 4 else target.shape[0]
+        cp = params._c()
+        ds = ctypes.c_int()
+        _check(self._lib.grn_add_dataset(self._h, ctypes.byref(cp), n_exp,
+                                         init.ctypes.data if init.size else None,
+                                         target.ctypes.data if target.size else None,
+                                         ctypes.byref(ds)))
+        self.params.append(params)
+        return ds.value
+
+    def evaluate(self, model: Model, max_error: float = float("inf"), dataset: int = 0) -> np.void:
+        """Plain path: one model per launch, synchronous (the reference's pattern)."""
+        cm = model._c()
+        r = _Result()
+        _check(self._lib.grn_evaluate(self._h, dataset, ctypes.byref(cm), max_error, ctypes.byref(r)))
+        out = np.zeros(1, RESULT_DTYPE)
+        ctypes.memmove(out.ctypes.data, ctypes.byref(r), RESULT_DTYPE.itemsize)
+        return out[0]
+
+    def evaluate_batch(self, models, datasets=None, max_errors=None, dt=None,
+                       path: int = PATH_AUTO) -> np.ndarray:
+        """Batched path: all models in persistent launches, one sync."""
+        n = len(models)
+        cms = (_Model * max(n, 1))(*[m._c() for m in models])
+        ds = None if datasets is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
+        me = None if max_errors is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(max_errors, np.float64), (n,)))
+        dts = None if dt is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+        out = np.zeros(max(n, 1), RESULT_DTYPE)
+        _check(self._lib.grn_evaluate_batch(
+            self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
+            None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data,
+            path, out.ctypes.data))
+        return out[:n]
+
+    def simulate_fields(self, model: Model, dataset: int = 0, experiment: int = 0,
+                        dt: float = 0.0, path: int = PATH_AUTO) -> np.ndarray:
+        """Final field [n_products][H][W] (float64) of one experiment."""
+        p = self.params[dataset]
+        cap = MAX_GENES * p.dim_x * p.dim_y
+        buf = np.zeros(cap, np.float64)
+        g = ctypes.c_int()
+        cm = model._c()
+        _check(self._lib.grn_simulate_fields(self._h, dataset, ctypes.byref(cm), dt, experiment, path,
+                                             buf.ctypes.data, cap, ctypes.byref(g)))
+        return buf[: g.value * p.dim_x * p.dim_y].reshape(g.value, p.dim_y, p.dim_x).copy()
+
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(self._lib.grn_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+
+def device_count() -> int:
+    n = ctypes.c_int()
+    _check(load_library().grn_device_count(ctypes.byref(n)))
+    return n.value

@@ -1,0 +1,197 @@
+/*
+ * grn.h -- C ABI of the MI355X-native GRN simulation-and-fitness library
+ * (libgrn_hip.so).  Plain C types only: no HIP, torch or C++ types cross it.
+ *
+ * Every entry point returns GRN_OK (0) or a negative grn_status; the message
+ * of the last failure on the calling thread is available from
+ * grn_last_error().  No C++ exception crosses the ABI.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   grn_create + grn_add_dataset
+ *       CEvaluatorMorphogens::CEvaluatorMorphogens(int threadId, const Search&)
+ *       Src/GPUSimulator/cevaluatormorphogens.h:23, .cpp:35-56 (device select,
+ *       stream) and createGPUExperimentDataset/copySimState (.cpp:92-191:
+ *       SimParams scalars + initial and target planes to the device)
+ *   grn_evaluate
+ *       double CEvaluatorMorphogens::evaluate(const Model&, double maxError)
+ *       cevaluatormorphogens.h:26, .cpp:233-245; with
+ *       CModelSimulator::loadModel (cmodelsimulator.cpp:114-227) and
+ *       CModelSimulator::simulateExperiment + launchKernel
+ *       (cmodelsimulator.cpp:229-242, modelsimulatordevice.cu:253-264):
+ *       one model per launch, synchronous, the reference's calling pattern
+ *   grn_evaluate_batch
+ *       no single reference call: it replaces the per-model loop that
+ *       ErrorCalculatorMultiThread's GPU workers run over one deme's queue
+ *       (Src/UI/Evolution/errorcalculatormultithread.cpp:175-199,248-254);
+ *       results are per-model identical to grn_evaluate
+ *   grn_simulate_fields
+ *       the final SimState a Simulator leaves behind (Src/Simulator/
+ *       simulator.cpp:54-63) -- used for per-cell parity checks
+ *   grn_destroy
+ *       CEvaluatorMorphogens::~CEvaluatorMorphogens / deleteGPUExperimentDatasets
+ *       (cevaluatormorphogens.cpp:58-61,193-231)
+ */
+#ifndef GRN_H
+#define GRN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRN_ABI_VERSION 1
+
+/* Capacity of the device program (compile-time limits of the kernels). */
+#define GRN_MAX_GENES 16  /* genes in use per model */
+#define GRN_MAX_LINKS 96  /* links into in-use genes per model */
+
+typedef enum {
+  GRN_OK = 0,
+  GRN_E_INVALID = -1,  /* bad argument */
+  GRN_E_MODEL = -2,    /* malformed model (missing product for a label) */
+  GRN_E_CAPACITY = -3, /* model or grid beyond the kernels' limits */
+  GRN_E_DEVICE = -4,   /* HIP runtime error */
+  GRN_E_NOMEM = -5,    /* host or device allocation failed */
+  GRN_E_RESULT = -6    /* the kernel did not write a result */
+} grn_status;
+
+typedef enum {
+  GRN_FITNESS_REF = 0, /* thresholded log error + non-convergence penalty
+                          (simstate.cpp:135-158, evaluatormorphogens.cpp:48-76) */
+  GRN_FITNESS_MSE = 1  /* mean squared error over cells and target planes */
+} grn_fitness_kind;
+
+/* Path selection for grn_evaluate_batch. */
+typedef enum {
+  GRN_PATH_AUTO = 0,  /* tuned per size class */
+  GRN_PATH_LDS = 1,   /* field resident on chip (registers + LDS halo) */
+  GRN_PATH_TILED = 2  /* field in device memory, halo-tiled */
+} grn_path;
+
+/* The SimParams fields the simulation and fitness read
+ * (Src/Simulator/simparams.h:25-43). */
+typedef struct {
+  int dim_x, dim_y;            /* size.width(), size.height() */
+  int n_input_morphogens;      /* nInputMorphogens */
+  int n_target_morphogens;     /* nTargetMorphogens */
+  int num_sim_steps;           /* NumSimSteps */
+  double dt;                   /* dt */
+  double min_conc;             /* minConc */
+  double min_conc_change;      /* minConcChange (< 0: run every step) */
+  double dist_error_threshold; /* distErrorThreshold */
+  int error_precision;         /* errorPrecision (ceilS digits; <= 0: none) */
+  int fitness_kind;            /* grn_fitness_kind */
+} grn_sim_params;
+
+/* ModelProd (Src/Model/modelprod.h:10-48). init is carried but unused by the
+ * simulation, as in the reference (modelprod.cpp:61-70). */
+typedef struct {
+  int label;
+  double init, lim, deg, dif;
+} grn_gene;
+
+/* ModelLink (Src/Model/modellink.h:10-46): hill_coef >= 0 activates,
+ * < 0 represses with Hill exponent |hill_coef|. */
+typedef struct {
+  int regulator, regulated; /* product labels */
+  double dis_const, hill_coef;
+  int is_and; /* AND (1) or OR (0) regulation */
+} grn_link;
+
+/* Model (Src/Model/model.h:14-91): caller-owned, read-only. */
+typedef struct {
+  int n_genes;
+  const grn_gene *genes;
+  int n_links;
+  const grn_link *links;
+} grn_model;
+
+typedef struct {
+  double fitness;   /* 100 if raw_error > max_error, else ceilS(raw_error) */
+  double raw_error; /* error accumulated over experiments */
+  double change;    /* accumulated last-step max |dc/dt| over experiments */
+  int n_products;   /* genes in use */
+  int steps;        /* Euler steps executed, summed over experiments */
+  int experiments;  /* experiments simulated before the max_error cut */
+  int reserved;
+} grn_result;
+
+typedef struct {
+  double kernel_ms;       /* device time of the last evaluate call (events) */
+  double host_ms;         /* wall time of the last evaluate call */
+  double cell_steps;      /* sum over models of W*H*steps executed */
+  double alg_flops;       /* sum over models of cell-steps * F(model) */
+  int n_launches;         /* kernel launches issued by the last call */
+  int n_models;
+} grn_stats;
+
+/* Library ABI version (GRN_ABI_VERSION). */
+int grn_abi_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *grn_last_error(void);
+
+/* Number of visible HIP devices. */
+int grn_device_count(int *n);
+
+/* Selects the device, creates the context's stream and device buffers. */
+int grn_create(int device, void **ctx);
+
+/* Frees everything the context owns. Safe on NULL. */
+void grn_destroy(void *ctx);
+
+/* Uploads one experiment set: params plus initial planes
+ * init[n_experiments][n_input_morphogens][dim_y][dim_x] and target planes
+ * target[n_experiments][n_target_morphogens][dim_y][dim_x] (x fastest).
+ * Returns the dataset id used by the evaluate calls. */
+int grn_add_dataset(void *ctx, const grn_sim_params *params, int n_experiments,
+                    const double *init, const double *target, int *dataset_id);
+
+/* Plain path: compile, launch one workgroup, synchronize -- one model per
+ * launch as the reference does. */
+int grn_evaluate(void *ctx, int dataset_id, const grn_model *model,
+                 double max_error, grn_result *out);
+
+/* Batched path: every model of the batch in persistent launches (one per
+ * size class), synchronizing once.  dataset_ids[i] selects model i's
+ * experiment set; dt_override (optional) replaces the dataset's dt for model
+ * i when > 0; max_errors (optional, default +inf). */
+int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
+                       const grn_model *models, const double *max_errors,
+                       const double *dt_override, int path, grn_result *out);
+
+/* Simulates experiment `experiment` of a dataset for one model and writes
+ * the final field, fields[n_products][dim_y][dim_x] as double, capacity in
+ * elements.  Runs the same kernel code as grn_evaluate. */
+int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model,
+                        double dt_override, int experiment, int path,
+                        double *fields, long capacity, int *n_products);
+
+/* Timing and work counters of the last evaluate call on this context. */
+int grn_get_stats(void *ctx, grn_stats *out);
+
+/* MathAlgo::ceilS (Src/Common/mathalgo.h:55-62). */
+double grn_ceil_s(double n, int digits);
+
+/* Model text format, Model::operator>> (Src/Model/model.cpp:659-706,
+ * modelprod.cpp:81-106, modellink.cpp:175-202):
+ *   (label init lim deg dif|...*regulator regulated K n isAnd|...)
+ * Writes up to gene_cap genes and link_cap links; *n_genes / *n_links get
+ * the model's counts even when the capacity is too small (GRN_E_CAPACITY). */
+int grn_model_parse(const char *text, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                    int link_cap, int *n_links);
+
+/* Model::operator<< (Src/Model/model.cpp:627-657): numbers with `precision`
+ * significant digits (<= 0: 6, the reference stream's default).  *len gets
+ * the text length without the terminating NUL. */
+int grn_model_format(const grn_model *model, int precision, char *buf, long cap, long *len);
+
+/* Compiles a model for a dataset with n_morphogens structural genes and
+ * reports genes in use, Hill links evaluated and the algorithmic FLOPs per
+ * cell-step, F = 5L + (L - G_h) + 8G + 7D. */
+int grn_model_info(const grn_model *model, int n_morphogens, int *genes_in_use, int *hill_links,
+                   double *flops_per_cell_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRN_H */

@@ -1,0 +1,104 @@
+"""CPU tests of the C ABI library: it loads, exports every entry point that
+include/grn.h declares, and its host logic (model text format, model
+compiler bookkeeping, ceilS) behaves.  No compute call is made here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import grn
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "grn.h")
+KAT = open(os.path.join(ROOT, "tests", "golden", "kat_model.txt")).read().strip()
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(grn_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(grn.LIB_PATH), "build libgrn_hip.so first"
+    out = subprocess.run(["nm", "-D", "--defined-only", grn.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (grn_\w+)", out))
+    missing = [f for f in declared_functions() if f not in exported]
+    assert not missing, missing
+    assert set(grn.EXPORTED) == set(declared_functions())
+
+
+def test_library_loads_and_reports_version():
+    lib = grn.load_library()
+    assert lib.grn_abi_version() == 1
+    assert lib.grn_last_error() is not None
+
+
+def test_ceil_s_matches_reference_values():
+    assert grn.ceil_s(0.0, 4) == 0.0
+    assert grn.ceil_s(1.23456, 3) == pytest.approx(1.24, abs=0)
+    assert grn.ceil_s(-1.23456, 3) == pytest.approx(-1.23, abs=0)
+    once = grn.ceil_s(0.048701011019999997, 10)
+    assert once == pytest.approx(0.04870101102, rel=1e-15)
+    assert grn.ceil_s(once, 10) == once
+
+
+def test_kat_model_parses_and_round_trips():
+    m = grn.Model.from_text(KAT)
+    assert len(m.genes) == 13 and len(m.links) == 19
+    assert set(m.genes["label"]) >= {0, 1}
+    assert m.genes["init"][m.genes["label"] == 0][0] == -1  # red morphogen tag
+    assert m.genes["init"][m.genes["label"] == 1][0] == -2  # green morphogen tag
+    text = m.to_text()
+    assert text == KAT  # six significant digits, as the reference writes them
+    again = grn.Model.from_text(text)
+    assert np.array_equal(again.genes, m.genes) and np.array_equal(again.links, m.links)
+    full = grn.Model.from_text(m.to_text(precision=17))
+    assert np.array_equal(full.genes, m.genes)
+
+
+def test_kat_model_in_use_and_flop_count():
+    m = grn.Model.from_text(KAT)
+    G, L, F = m.info(n_morphogens=2)
+    # in use: 0, 1, 64653, 107652 (the reference calls it a 4-product model)
+    assert G == 4
+    # Hill links into in-use genes with an activator: gene 0 <- 64653 (act);
+    # gene 1 <- 0 (act), 1 (act), 107652 (rep); 64653 <- 0 (act);
+    # 107652 <- 1 (act), 64653 (rep)
+    assert L == 7
+    G_h, D = 4, 1  # every in-use gene has an activator; only label 0 diffuses
+    assert F == 5 * L + (L - G_h) + 8 * G + 7 * D
+
+
+@pytest.mark.parametrize("bad", ["", "(", "(0 1 2 3|)", "(0 0 1 1 0*0 0 1)", "0 0 1 1 0*)"])
+def test_malformed_text_is_rejected(bad):
+    with pytest.raises(grn.GrnError):
+        grn.Model.from_text(bad)
+
+
+def test_model_without_links_parses():
+    m = grn.Model.from_text("(0 -1 0.5 0.2 0|1 -2 0.1 0.3 0.05*)")
+    assert len(m.genes) == 2 and len(m.links) == 0
+    assert m.to_text() == "(0 -1 0.5 0.2 0|1 -2 0.1 0.3 0.05*)"
+
+
+def test_compile_rejects_missing_product_and_capacity():
+    # label 7 regulates gene 0 but has no product
+    m = grn.Model.from_text("(0 0 0.5 0.2 0|1 0 0.1 0.3 0*7 0 0.5 2 0)")
+    with pytest.raises(grn.GrnError) as e:
+        m.info(2)
+    assert e.value.code == -2
+    # 17 genes in use exceed the kernels' 16
+    genes = np.zeros(17, grn.GENE_DTYPE)
+    genes["label"] = np.arange(17)
+    links = np.zeros(16, grn.LINK_DTYPE)
+    links["regulator"] = np.arange(1, 17)
+    links["regulated"] = 0
+    links["dis_const"] = 0.5
+    links["hill_coef"] = 1.0
+    with pytest.raises(grn.GrnError) as e:
+        grn.Model(genes, links).info(1)
+    assert e.value.code == -3

@@ -1,0 +1,198 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Tolerances (FP32 compensated state on the device vs FP64 on the host):
+  * fitness (raw error):   |gpu - cpu| <= 1e-5 * |cpu|
+  * final field per cell:  |gpu - cpu| <= 1e-4 * |cpu| + 1e-7
+  * ranking of a population by fitness: identical order
+Integer outputs (steps executed, experiments run, genes in use) are exact.
+"""
+import numpy as np
+import pytest
+
+import grn
+import grn_oracle as ora
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+FIT_RTOL = 1e-5
+FIELD_RTOL, FIELD_ATOL = 1e-4, 1e-7
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    print("\n[gpu] creating context on device 0", flush=True)
+    c = grn.Context(0)
+    yield c
+    c.close()
+
+
+def check_fitness(got, want):
+    err = np.abs(got - want)
+    assert np.all(err <= FIT_RTOL * np.abs(want) + 1e-300), (
+        f"max rel err {np.max(err / np.maximum(np.abs(want), 1e-300)):.3e}")
+
+
+def check_fields(got, want):
+    assert got.shape == want.shape
+    bad = np.abs(got - want) > FIELD_RTOL * np.abs(want) + FIELD_ATOL
+    assert not bad.any(), (f"{bad.sum()} cells out of tolerance, worst abs {np.abs(got - want).max():.3e}")
+
+
+def population_case(seed, n_models, W, H, S, n_in=2, n_tgt=1, g_range=(3, 3), l_range=(2, 6),
+                    kind=grn.FITNESS_MSE, mcc=-1.0, thr=0.0, dt=0.01):
+    rng = np.random.default_rng(seed)
+    p = grn.SimParams(W, H, n_in, n_tgt, S, dt, min_conc_change=mcc, dist_error_threshold=thr,
+                      fitness_kind=kind)
+    planes = [workloads.gradient_x(W, H), workloads.gradient_y(W, H),
+              workloads.gaussian_blob(W, H, rng.uniform(0, W - 1), rng.uniform(0, H - 1), max(W, H) / 8)]
+    init = np.stack([planes[i % 3] for i in range(n_in)])[None]
+    target = rng.uniform(0, 1, (1, n_tgt, H, W))
+    models = workloads._population(rng, n_models, g_range, l_range, n_in + n_tgt)
+    return p, init, target, models
+
+
+def run_both(ctx, p, init, target, models, max_errors=None, n_threads=8):
+    print(f"[gpu] batch of {len(models)} on {p.dim_x}x{p.dim_y}, {p.num_sim_steps} steps", flush=True)
+    ds = ctx.add_dataset(p, init, target)
+    got = ctx.evaluate_batch(models, datasets=ds, max_errors=max_errors)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, max_errors=max_errors,
+                              n_threads=n_threads)
+    return ds, got, want
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_config0_population_fitness_and_ranking(ctx, seed):
+    w = workloads.config0(seed=seed)
+    workloads.make_target(ctx, w)
+    ds, got, want = run_both(ctx, w.params, w.init, w.target, w.models)
+    assert np.all(got["steps"] == 1000) and np.all(want["steps"] == 1000)
+    assert np.array_equal(got["n_products"], want["n_products"])
+    check_fitness(got["raw_error"], want["raw_error"])
+    assert np.array_equal(np.argsort(got["raw_error"], kind="stable"),
+                          np.argsort(want["raw_error"], kind="stable"))
+
+
+def test_config0_fields_per_cell(ctx):
+    w = workloads.config0(seed=3, n_models=8)
+    workloads.make_target(ctx, w)
+    ds = ctx.add_dataset(w.params, w.init, w.target)
+    op = ora.from_simparams(w.params)
+    for i, m in enumerate(w.models):
+        print(f"[gpu] fields model {i}", flush=True)
+        f = ctx.simulate_fields(m, dataset=ds)
+        _, fo = ora.evaluate(op, w.init, w.target, m.genes, m.links, want_fields=True)
+        check_fields(f, fo)
+
+
+def test_plain_equals_batched_bitwise(ctx):
+    p, init, target, models = population_case(7, 24, 32, 32, 300, g_range=(3, 9), l_range=(1, 30))
+    ds = ctx.add_dataset(p, init, target)
+    batch = ctx.evaluate_batch(models, datasets=ds)
+    for i in range(0, len(models), 5):
+        one = ctx.evaluate(models[i], dataset=ds)
+        assert one["raw_error"] == batch[i]["raw_error"]
+        assert one["steps"] == batch[i]["steps"]
+    again = ctx.evaluate_batch(models[::-1], datasets=ds)[::-1]
+    assert np.array_equal(again["raw_error"], batch["raw_error"])
+
+
+@pytest.mark.parametrize("W,H,gr", [(32, 32, (3, 16)), (48, 48, (3, 12)), (64, 64, (4, 9)),
+                                    (16, 16, (3, 8)), (20, 12, (3, 8)), (7, 5, (3, 16))])
+def test_grid_shapes_and_gene_counts(ctx, W, H, gr):
+    p, init, target, models = population_case(11 + W, 16, W, H, 150, n_in=3, g_range=gr, l_range=(1, 40))
+    ds, got, want = run_both(ctx, p, init, target, models)
+    assert np.array_equal(got["n_products"], want["n_products"])
+    check_fitness(got["raw_error"], want["raw_error"])
+
+
+def test_ref_fitness_with_early_stop_penalty_and_rounding(ctx):
+    p, init, target, models = population_case(21, 24, 32, 32, 3000, g_range=(3, 7), l_range=(1, 12),
+                                              kind=grn.FITNESS_REF, mcc=1e-3, thr=0.05, dt=0.05)
+    p.error_precision = 6
+    ds, got, want = run_both(ctx, p, init, target, models)
+    # the stop step is an integer decision on a float threshold: FP32 vs FP64
+    # may differ by a step only when the change sits on the threshold
+    assert np.mean(got["steps"] == want["steps"]) >= 0.9
+    same = got["steps"] == want["steps"]
+    check_fitness(got["raw_error"][same], want["raw_error"][same])
+    assert np.any(got["steps"][same] < 3000)  # some models do stop early
+
+
+def test_multiple_experiments_and_max_error_cut(ctx):
+    W = H = 32
+    rng = np.random.default_rng(5)
+    p = grn.SimParams(W, H, 2, 1, 200, 0.02, fitness_kind=grn.FITNESS_MSE)
+    init = rng.uniform(0, 1, (3, 2, H, W))
+    target = rng.uniform(0, 1, (3, 1, H, W))
+    models = workloads._population(rng, 16, (3, 6), (2, 10), 3)
+    ds, got0, want0 = run_both(ctx, p, init, target, models)
+    check_fitness(got0["raw_error"], want0["raw_error"])
+    assert np.all(got0["experiments"] == 3)
+    cut = np.median(want0["raw_error"]) * np.ones(len(models))
+    got = ctx.evaluate_batch(models, datasets=ds, max_errors=cut)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, max_errors=cut, n_threads=8)
+    assert np.array_equal(got["experiments"], want["experiments"])
+    assert np.array_equal(got["fitness"] == 100, want["fitness"] == 100)
+
+
+def test_negative_inputs_take_guarded_kernel(ctx):
+    # negative input values make the diffusion guard of modelsimulator.cpp:255
+    # matter; min_conc < 0 keeps negative concentrations alive
+    W, H = 24, 16
+    rng = np.random.default_rng(9)
+    p = grn.SimParams(W, H, 2, 1, 200, 0.02, min_conc=-10.0, fitness_kind=grn.FITNESS_MSE)
+    init = rng.uniform(-0.5, 1, (1, 2, H, W))
+    init[0, 0, :, :3] = 0.0  # zero patches where the guard switches diffusion off
+    target = rng.uniform(0, 1, (1, 1, H, W))
+    # no links: h = 0, so negative concentrations never reach a Hill term
+    models = []
+    for i in range(8):
+        g = np.zeros(3, grn.GENE_DTYPE)
+        g["label"] = [0, 1, 2]
+        g["lim"] = rng.uniform(0, 1, 3)
+        g["deg"] = rng.uniform(0.01, 1, 3)
+        g["dif"] = rng.uniform(0, 0.2, 3)
+        models.append(grn.Model(g, np.zeros(0, grn.LINK_DTYPE)))
+    ds, got, want = run_both(ctx, p, init, target, models)
+    check_fitness(got["raw_error"], want["raw_error"])
+    f = ctx.simulate_fields(models[0], dataset=ds)
+    _, fo = ora.evaluate(ora.from_simparams(p), init, target, models[0].genes, models[0].links,
+                         want_fields=True)
+    check_fields(f, fo)
+
+
+def test_kat_model_evaluates(ctx):
+    m = grn.Model.from_text(open(__file__.replace("test_gpu_parity.py", "golden/kat_model.txt")).read())
+    W = H = 32
+    # labels 0 and 1 are the structural morphogens (init tags -1, -2)
+    p = grn.SimParams(W, H, 1, 1, 500, 0.01, min_conc_change=-1, dist_error_threshold=0.01,
+                      error_precision=10, fitness_kind=grn.FITNESS_REF)
+    init = workloads.gradient_x(W, H)[None, None]
+    target = workloads.gradient_y(W, H)[None, None]
+    ds, got, want = run_both(ctx, p, init, target, [m])
+    check_fitness(got["raw_error"], want["raw_error"])
+    assert got[0]["n_products"] == 4
+
+
+def test_config1_batch_invariance_and_determinism(ctx):
+    # full config-1 shape (64x64, 2000 steps) on a 512-model slice: results
+    # do not depend on batch composition or order, and repeat bit for bit
+    w = workloads.config1(seed=0, n_models=512)
+    workloads.make_target(ctx, w)
+    ds = ctx.add_dataset(w.params, w.init, w.target)
+    print("[gpu] config1 slice batch", flush=True)
+    a = ctx.evaluate_batch(w.models, datasets=ds)
+    b = ctx.evaluate_batch(w.models, datasets=ds)
+    assert np.array_equal(a["raw_error"], b["raw_error"])
+    sub = list(range(0, 512, 37))
+    c = ctx.evaluate_batch([w.models[i] for i in sub], datasets=ds)
+    assert np.array_equal(c["raw_error"], a["raw_error"][sub])
+    assert np.all(a["steps"] == 2000) and np.all(np.isfinite(a["raw_error"])) and np.all(a["raw_error"] >= 0)
+
+
+def test_config1_parity_sample(ctx):
+    w = workloads.config1(seed=1, n_models=16)
+    workloads.make_target(ctx, w)
+    ds, got, want = run_both(ctx, w.params, w.init, w.target, w.models)
+    check_fitness(got["raw_error"], want["raw_error"])
