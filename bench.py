@@ -1,0 +1,150 @@
+"""Benchmark: GRN cell-steps/s of the batched on-chip path on one MI355X.
+
+Workload (one "step" = one batched evaluation of the whole population):
+4096 random GRN models with 3-8 genes and 2-24 links on a 64x64 grid, 2000
+explicit-Euler steps with dt = 0.01, FP32 on the device, MSE fitness against
+the output of a hidden 6-gene GRN; inputs are an x-gradient, a y-gradient and a
+Gaussian blob.  Synthetic, seeded data.
+
+Prints one JSON line.  Besides the contract fields it carries:
+  roofline      achieved FP32 FLOP/s of the field kernel (algorithmic count
+                F = 5L + (L - G_h) + 8G + 7D per cell-step, over the kernel
+                time measured with HIP events on the library's stream)
+                against the 157.3 TFLOP/s FP32 vector peak
+  plain         the reference's calling pattern (one model per launch, one
+                synchronisation per model) on a sample of the same models
+  cpu_baseline  the FP64 CPU oracle on the host cores, bounded sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "grn-gpu-performance_amd"))
+
+import numpy as np  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, budget_s=15.0):
+    """FP64 oracle on the host cores over a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import grn_oracle as ora
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    p = ora.from_simparams(w.params)
+    cells = w.params.dim_x * w.params.dim_y
+    # calibrate on one model per thread, then size the timed sample
+    probe = w.models[:threads]
+    t0 = time.perf_counter()
+    r = ora.evaluate_batch(p, w.init, w.target, probe, n_threads=threads)
+    dt = time.perf_counter() - t0
+    n = int(np.clip(len(probe) * budget_s / max(dt, 1e-3), threads, len(w.models)))
+    n = max(threads, n // threads * threads)
+    sample = w.models[len(probe):len(probe) + n]
+    t0 = time.perf_counter()
+    r = ora.evaluate_batch(p, w.init, w.target, sample, n_threads=threads)
+    dt = time.perf_counter() - t0
+    cs = float(np.sum(r["steps"]) * cells)
+    return {"value": cs / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} of the {len(w.models)} models at full 2000 steps, FP64, "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--models", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(models shard across GPUs by island, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import grn
+    import workloads
+
+    w = workloads.config1(seed=args.seed, n_models=args.models)
+    ctx = grn.Context(0)
+    workloads.make_target(ctx, w)
+    ds = ctx.add_dataset(w.params, w.init, w.target)
+    cells = w.params.dim_x * w.params.dim_y
+    flops_per_model = np.array([m.info(4)[2] for m in w.models])
+
+    log(f"[bench] warmup x{args.warmup}")
+    for _ in range(args.warmup):
+        ctx.evaluate_batch(w.models, datasets=ds)
+    log(f"[bench] timing x{args.steps}")
+    kernel_ms, host_ms, cell_steps, alg_flops = [], [], 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = ctx.evaluate_batch(w.models, datasets=ds)
+        st = ctx.stats()
+        kernel_ms.append(st["kernel_ms"])
+        host_ms.append(st["host_ms"])
+        cell_steps += st["cell_steps"]
+        alg_flops += st["alg_flops"]
+    wall = time.perf_counter() - t0
+    assert np.all(res["steps"] == w.params.num_sim_steps)
+    value = cell_steps / wall
+    k_avg = float(np.mean(kernel_ms)) / 1e3
+    achieved = (alg_flops / args.steps) / k_avg / 1e12
+
+    log(f"[bench] plain path on {args.plain_sample} models")
+    sample = w.models[: args.plain_sample]
+    t0 = time.perf_counter()
+    for m in sample:
+        r = ctx.evaluate(m, dataset=ds)
+    plain_wall = time.perf_counter() - t0
+    plain_rate = len(sample) * cells * w.params.num_sim_steps / plain_wall
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(w)
+
+    out = {
+        "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "batched GRN simulation+fitness, config 1: 4096 models, 3-8 genes, "
+                               "2-24 links, 64x64 grid, 2000 Euler steps, dt 0.01, MSE vs hidden 6-gene GRN",
+                   "models": args.models, "grid": "64x64", "sim_steps": w.params.num_sim_steps,
+                   "parallelism": "one persistent launch per batch"},
+        "kernel_cell_steps_per_s": cell_steps / args.steps / k_avg,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": float(np.mean(kernel_ms)),
+                     "flops_per_cell_step_mean": float(np.mean(flops_per_model))},
+        "plain": {"value": plain_rate, "unit": "cell-steps/s", "models": len(sample),
+                  "pattern": "one model per launch, synchronise per model"},
+        "batched_over_plain": value / plain_rate,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -7,9 +7,9 @@
 //   * every thread owns a vertical strip of CPT cells (column x, rows
 //     y0..y0+CPT-1) and keeps all G genes of those cells in registers as a
 //     compensated pair (hi, lo) of FP32 values;
-//   * every gene's plane is mirrored in LDS with a zero border, so a step
-//     reads east/west neighbours and the two strip-end neighbours from LDS and
-//     the in-strip north/south neighbours from registers;
+//   * every gene's plane is mirrored in LDS with a zero border, from which a
+//     step reads the four neighbours (two ds_read2 per gene and cell), so
+//     only the owning thread's registers hold the state;
 //   * Hill regulation is evaluated in exponent form (grn_program.h): one log2
 //     per gene, one FMA + min/max per link, one exp2 and one reciprocal per
 //     gene;
@@ -36,7 +36,13 @@ namespace grn_field {
 // Per-cell log2 values, indexed by a uniform regulator index.  Sixteen lanes
 // keep the vector above the size at which the compiler turns a dynamic index
 // into a select chain, so each read is one indexed register move.
-typedef float lg_vec __attribute__((ext_vector_type(16)));
+// Per-cell log2 values, indexed by a uniform regulator index.  Built with
+// -amdgpu-use-divergent-register-indexing so that every width is read with
+// one indexed register move (s_set_gpr_idx) rather than a select chain.
+template <int G>
+struct LgVec {
+  typedef float T __attribute__((ext_vector_type(G <= 4 ? 4 : (G <= 8 ? 8 : 16))));
+};
 
 __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -70,7 +76,7 @@ struct Model {
   const GrnProgram *P;
   int W, H, plane;
   float dt, min_conc;
-  bool clamp_nonzero_r;  // clamp only where R != 0 (needed when min_conc > 0)
+  bool clamp_any_r;      // the R != 0 test of the clamp is moot (min_conc <= 0, field >= 0)
   bool active;
   int x, y0, ubase;      // ubase: LDS index of the cell north-west of cell k=0
   float nf[CPT];         // neighbour count per cell
@@ -93,13 +99,12 @@ struct Model {
   // strip.  With TRACK the workgroup-wide max |R| of the step is returned.
   template <bool TRACK>
   __device__ __forceinline__ float step() {
-    float prev[G];  // old values of the cell above (k-1)
     float cmax = 0.f;
 
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const GrnProgram *Pq = P + opaque_zero();
-      lg_vec lg;  // lanes >= G are never read
+      typename LgVec<G>::T lg;  // lanes >= G are never read
 #pragma unroll
       for (int g = 0; g < G; ++g) lg[g] = __builtin_fmaxf(hw_log2(hi[g][k]), -kMaxF);
 
@@ -124,8 +129,8 @@ struct Model {
         const float *pl = sh.lds + p * plane + ubase;
         const float left = pl[(k + 1) * pc()];
         const float right = pl[(k + 1) * pc() + 2];
-        const float up = (k == 0) ? pl[1] : prev[p];
-        const float down = (k == CPT - 1) ? pl[(CPT + 1) * pc() + 1] : hi[p][k + 1];
+        const float up = pl[k * pc() + 1];
+        const float down = pl[(k + 2) * pc() + 1];
         const float S = ((left + right) + up) + down;
         float lap = __builtin_fmaf(-nf[k], c, S);
         if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
@@ -135,15 +140,18 @@ struct Model {
         R = __builtin_fmaf(gp.dif, lap, R);
 
         // Compensated Euler update c += dt*R (Fast2Sum on the pair) with the
-        // minConc clamp of modelsimulator.cpp:263-275.
+        // minConc clamp of modelsimulator.cpp:263-275.  The empty asm takes
+        // the results here, so the compiler cannot sink this gene's update
+        // below the next genes' link loops (which keeps every gene's
+        // constants and partial sums alive at once).
         const float yv = __builtin_fmaf(dt, R, lo[p][k]);
         const float sv = c + yv;
         const float nl = yv - (sv - c);
-        const bool clamp = clamp_nonzero_r ? ((sv < min_conc) && (R != 0.f)) : (sv < min_conc);
+        const bool clamp = (sv < min_conc) && (clamp_any_r || R != 0.f);
         hi[p][k] = clamp ? 0.f : sv;
         lo[p][k] = clamp ? 0.f : nl;
+        asm volatile("" : "+v"(hi[p][k]), "+v"(lo[p][k]));
         if (TRACK) cmax = __builtin_fmaxf(cmax, __builtin_fabsf(R));
-        prev[p] = c;
       }
     }
 
@@ -187,6 +195,26 @@ struct Model {
   }
 };
 
+// Sum of the per-cell fitness terms over this thread's share of the staged
+// final planes: REF terms log(1 + (|c - T| - thr)) where positive
+// (simstate.cpp:135-158), MSE terms (c - T)^2.  Kept out of line so its
+// double-precision log does not share a register allocation with the
+// simulation loop.
+__device__ __noinline__ double score_terms(const double *stage, const double *target, int count,
+                                           double thr, int mse) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const double d = stage[i] - target[i];
+    if (mse) {
+      acc += d * d;
+    } else {
+      const double u = __builtin_fabs(d) - thr;
+      if (u > 0) acc += log(1.0 + u);
+    }
+  }
+  return acc;
+}
+
 // Simulates and scores one model (all experiments) -- the body of the
 // reference's simulateModelKernel (modelsimulatordevice.cu:216-251) with the
 // CPU semantics of EvaluatorMorphogens::evaluate.
@@ -206,7 +234,7 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
   f.plane = f.pc() * (f.H + 2);
   f.dt = P->dt;
   f.min_conc = D->min_conc;
-  f.clamp_nonzero_r = D->min_conc > 0.f || !D->nonneg;
+  f.clamp_any_r = !(D->min_conc > 0.f || !D->nonneg);
   const int W = f.W, H = f.H, N = W * H;
   const int n_active = W * (H / CPT);  // the launcher guarantees H % CPT == 0
   f.active = tid < n_active;
@@ -275,18 +303,7 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
     // REF: sum of log(1 + (|c - T| - thr)) over positive terms, / (W*H)
     // (simstate.cpp:135-158).  MSE: mean of (c - T)^2 over cells and targets.
     const bool mse = D->kind == 1;
-    const double thr = D->thr;
-    const double *tp = D->target + (size_t)e * n_tgt * N;
-    double acc = 0.0;
-    for (int i = tid; i < n_tgt * N; i += blockDim.x) {
-      const double d = stage[i] - tp[i];
-      if (mse) {
-        acc += d * d;
-      } else {
-        const double u = __builtin_fabs(d) - thr;
-        if (u > 0) acc += log(1.0 + u);
-      }
-    }
+    double acc = score_terms(stage, D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr, mse ? 1 : 0);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) sh.wsum[wave] = acc;

@@ -126,8 +126,14 @@ def evaluate_batch(p: _Params, init, target, models, max_errors=None, dt=None, n
     for i, m in enumerate(models):
         gene_off[i + 1] = gene_off[i] + len(m.genes)
         link_off[i + 1] = link_off[i] + len(m.links)
-    genes = np.ascontiguousarray(np.concatenate([np.asarray(m.genes).astype(GENE_DTYPE) for m in models]))
-    links = np.ascontiguousarray(np.concatenate([np.asarray(m.links).astype(LINK_DTYPE) for m in models]))
+    # fill preallocated arrays: np.concatenate would drop the dtype's padding
+    genes = np.zeros(max(int(gene_off[-1]), 1), GENE_DTYPE)
+    links = np.zeros(max(int(link_off[-1]), 1), LINK_DTYPE)
+    for i, m in enumerate(models):
+        for name in GENE_DTYPE.names:
+            genes[name][gene_off[i]:gene_off[i + 1]] = np.asarray(m.genes)[name]
+        for name in LINK_DTYPE.names:
+            links[name][link_off[i]:link_off[i + 1]] = np.asarray(m.links)[name]
     me = None if max_errors is None else np.ascontiguousarray(
         np.broadcast_to(np.asarray(max_errors, np.float64), (len(models),)))
     dts = None if dt is None else np.ascontiguousarray(
@@ -137,7 +143,7 @@ def evaluate_batch(p: _Params, init, target, models, max_errors=None, dt=None, n
     rc = load().gro_evaluate_batch_dt(ctypes.byref(p), n_exp, init.ctypes.data if init.size else None,
                                       target.ctypes.data if target.size else None, len(models),
                                       gene_off.ctypes.data, genes.ctypes.data, link_off.ctypes.data,
-                                      links.ctypes.data if len(links) else None,
+                                      links.ctypes.data,
                                       None if me is None else me.ctypes.data,
                                       None if dts is None else dts.ctypes.data, n_threads,
                                       out.ctypes.data)

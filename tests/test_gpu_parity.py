@@ -2,6 +2,9 @@
 
 Tolerances (FP32 compensated state on the device vs FP64 on the host):
   * fitness (raw error):   |gpu - cpu| <= 1e-5 * |cpu|
+    except the REF-mode non-convergence penalty (minConcChange crossing,
+    evaluatormorphogens.cpp:62-63), whose input -- the last step's max
+    |dc/dt| -- is an FP32 residual: that component gets 2e-4 relative
   * final field per cell:  |gpu - cpu| <= 1e-4 * |cpu| + 1e-7
   * ranking of a population by fitness: identical order
 Integer outputs (steps executed, experiments run, genes in use) are exact.
@@ -115,8 +118,15 @@ def test_ref_fitness_with_early_stop_penalty_and_rounding(ctx):
     # may differ by a step only when the change sits on the threshold
     assert np.mean(got["steps"] == want["steps"]) >= 0.9
     same = got["steps"] == want["steps"]
-    check_fitness(got["raw_error"][same], want["raw_error"][same])
+    # split the error into the log-distance part and the penalty part
+    pen_w = np.where(want["change"] > p.min_conc_change, want["change"] - p.min_conc_change, 0.0)
+    pen_g = np.where(got["change"] > p.min_conc_change, got["change"] - p.min_conc_change, 0.0)
+    dist_w, dist_g = want["raw_error"] - pen_w, got["raw_error"] - pen_g
+    check_fitness(dist_g[same], dist_w[same])
+    assert np.all(np.abs(pen_g - pen_w)[same] <= 2e-4 * pen_w[same] + 1e-12)
+    assert np.all(np.abs(got["change"] - want["change"])[same] <= 2e-4 * want["change"][same])
     assert np.any(got["steps"][same] < 3000)  # some models do stop early
+    assert np.any(pen_w[same] > 0)            # and some pay the penalty
 
 
 def test_multiple_experiments_and_max_error_cut(ctx):
@@ -129,7 +139,10 @@ def test_multiple_experiments_and_max_error_cut(ctx):
     ds, got0, want0 = run_both(ctx, p, init, target, models)
     check_fitness(got0["raw_error"], want0["raw_error"])
     assert np.all(got0["experiments"] == 3)
-    cut = np.median(want0["raw_error"]) * np.ones(len(models))
+    # cut half-way between two distinct errors, away from any tie
+    vals = np.unique(want0["raw_error"])
+    mid = len(vals) // 2
+    cut = 0.5 * (vals[mid - 1] + vals[mid]) * np.ones(len(models))
     got = ctx.evaluate_batch(models, datasets=ds, max_errors=cut)
     want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, max_errors=cut, n_threads=8)
     assert np.array_equal(got["experiments"], want["experiments"])

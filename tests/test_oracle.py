@@ -293,3 +293,16 @@ def test_kat_model_in_use_closure():
     import grn
     m = grn.Model.from_text(open(__file__.replace("test_oracle.py", "golden/kat_model.txt")).read().strip())
     assert list(ora.labels_in_use(m.genes.astype(G_DT), m.links.astype(L_DT), 2)) == [0, 1, 64653, 107652]
+
+
+def test_batch_matches_single_model_calls():
+    # the batch binding packs models into one gene / link array; a packing
+    # mistake (e.g. a dropped dtype padding) shows up as a different result
+    import workloads
+    w = workloads.config0(seed=4, n_models=6)
+    w.params.num_sim_steps = 20
+    p = ora.from_simparams(w.params)
+    tgt = np.random.default_rng(0).uniform(0, 1, (1, 1, 32, 32))
+    batch = ora.evaluate_batch(p, w.init, tgt, w.models, n_threads=3)
+    single = [ora.evaluate(p, w.init, tgt, m.genes, m.links)[0]["raw_error"] for m in w.models]
+    assert np.array_equal(batch["raw_error"], single)
