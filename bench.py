@@ -27,6 +27,11 @@ sys.path.insert(0, os.path.join(ROOT, "grn-gpu-performance_amd"))
 import numpy as np  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per batched launch of this workload: FETCH_SIZE x 2.0 + WRITE_SIZE
+# x 1.0 (scale factors from the copy-kernel calibration) as measured by
+# rocprofv3 --pmc; the commands and raw counts are in
+# profiles/field_kernel_config1_64x64.md.
+MEASURED_TRAFFIC_BYTES = 667.8e6
 
 
 def log(msg):
@@ -67,6 +72,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device with the library's "
+                         "copy kernel (known size, to calibrate FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -104,11 +112,18 @@ def main():
 
     log(f"[bench] plain path on {args.plain_sample} models")
     sample = w.models[: args.plain_sample]
-    t0 = time.perf_counter()
-    for m in sample:
-        r = ctx.evaluate(m, dataset=ds)
-    plain_wall = time.perf_counter() - t0
-    plain_rate = len(sample) * cells * w.params.num_sim_steps / plain_wall
+    plain_rate = None
+    if sample:
+        t0 = time.perf_counter()
+        for m in sample:
+            ctx.evaluate(m, dataset=ds)
+        plain_wall = time.perf_counter() - t0
+        plain_rate = len(sample) * cells * w.params.num_sim_steps / plain_wall
+
+    if args.copy_probe_mib:
+        ms = grn.copy_probe(args.copy_probe_mib << 20)
+        log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + written in {ms:.3f} ms "
+            f"({2 * (args.copy_probe_mib << 20) / ms / 1e6:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:
@@ -134,12 +149,13 @@ def main():
                    "parallelism": "one persistent launch per batch"},
         "kernel_cell_steps_per_s": cell_steps / args.steps / k_avg,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC_BYTES if args.models == 4096 else None,
                      "kernel_ms": float(np.mean(kernel_ms)),
                      "flops_per_cell_step_mean": float(np.mean(flops_per_model))},
         "plain": {"value": plain_rate, "unit": "cell-steps/s", "models": len(sample),
                   "pattern": "one model per launch, synchronise per model"},
-        "batched_over_plain": value / plain_rate,
+        "batched_over_plain": value / plain_rate if plain_rate else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

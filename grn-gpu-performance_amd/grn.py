@@ -39,6 +39,7 @@ EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
     "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
     "grn_get_stats", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
+    "grn_copy_probe",
 ]
 
 
@@ -114,6 +115,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_model_format.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, ctypes.c_char_p,
                                      ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     lib.grn_model_info.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, ip, ip, dp]
+    lib.grn_copy_probe.argtypes = [ctypes.c_long, dp]
     for name in EXPORTED:
         if name not in ("grn_last_error", "grn_destroy", "grn_ceil_s", "grn_abi_version"):
             getattr(lib, name).restype = ctypes.c_int
@@ -284,6 +286,13 @@ class Context:
         s = _Stats()
         _check(self._lib.grn_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+
+def copy_probe(nbytes: int) -> float:
+    """Runs the calibration copy kernel over nbytes; returns its time in ms."""
+    ms = ctypes.c_double()
+    _check(load_library().grn_copy_probe(nbytes, ctypes.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

@@ -169,6 +169,11 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model,
 /* Timing and work counters of the last evaluate call on this context. */
 int grn_get_stats(void *ctx, grn_stats *out);
 
+/* No reference counterpart: copies `bytes` (a multiple of 16) device to
+ * device with a plain copy kernel on the current device and returns its
+ * time.  Profilers use it to calibrate HBM byte counters. */
+int grn_copy_probe(long bytes, double *ms);
+
 /* MathAlgo::ceilS (Src/Common/mathalgo.h:55-62). */
 double grn_ceil_s(double n, int digits);
 
