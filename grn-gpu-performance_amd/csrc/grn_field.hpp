@@ -12,7 +12,10 @@
 //     only the owning thread's registers hold the state;
 //   * Hill regulation is evaluated in exponent form (grn_program.h): one log2
 //     per gene, one FMA + min/max per link, one exp2 and one reciprocal per
-//     gene;
+//     gene; each pass over a gene's links serves a block of CB cells of the
+//     strip at once, which amortises the per-link scalar work (load, register
+//     index set-up, loop control) -- the scalar pipe, not the vector ALU,
+//     limits a one-cell-per-pass loop;
 //   * the per-step max |dc/dt| (the reference's convergence test) and the
 //     final fitness are reduced in the same kernel.
 // The gene count G is a template parameter: the kernel switches once per
@@ -71,7 +74,7 @@ struct Shared {
 };
 
 // Everything one model needs; G, CPT, WC compile-time.
-template <int WC, int G, int CPT, bool GUARD>
+template <int WC, int G, int CPT, int NTMAX, bool GUARD>
 struct Model {
   const GrnProgram *P;
   int W, H, plane;
@@ -97,61 +100,95 @@ struct Model {
 
   // One Euler step of every owned cell, processed cell by cell down the
   // strip.  With TRACK the workgroup-wide max |R| of the step is returned.
+  // Cells per link pass: every link iteration serves CB cells, which
+  // amortises its scalar work (load, index set-up, loop control); CB is the
+  // largest divisor of CPT whose log2 vectors still fit the register budget
+  // of NTMAX threads next to the (hi, lo) state.
+  static constexpr int lg_width() { return G <= 4 ? 4 : (G <= 8 ? 8 : 16); }
+  static constexpr int budget() { return 131072 / NTMAX < 512 ? 131072 / NTMAX : 512; }
+  static constexpr bool fits(int cb) { return 2 * G * CPT + lg_width() * cb + 4 * cb + 32 <= budget(); }
+  static constexpr int cell_block() {
+    int best = 1;
+    for (int cb = 1; cb <= CPT; ++cb)
+      if (CPT % cb == 0 && fits(cb)) best = cb;
+    return best;
+  }
+
   template <bool TRACK>
   __device__ __forceinline__ float step() {
+    typedef typename LgVec<G>::T lg_t;
+    constexpr int CB = cell_block();
     float cmax = 0.f;
 
 #pragma unroll
-    for (int k = 0; k < CPT; ++k) {
+    for (int k0 = 0; k0 < CPT; k0 += CB) {
+      // Reload the program through an opaque offset in every block, so its
+      // constants are not kept in scalar registers across blocks.
       const GrnProgram *Pq = P + opaque_zero();
-      typename LgVec<G>::T lg;  // lanes >= G are never read
+      // log2 of every gene of this block's cells (their step-start values:
+      // a cell's update touches only that cell)
+      lg_t lg[CB];
 #pragma unroll
-      for (int g = 0; g < G; ++g) lg[g] = __builtin_fmaxf(hw_log2(hi[g][k]), -kMaxF);
+      for (int j = 0; j < CB; ++j)
+#pragma unroll
+        for (int g = 0; g < G; ++g) lg[j][g] = __builtin_fmaxf(hw_log2(hi[g][k0 + j]), -kMaxF);
 
 #pragma unroll
       for (int p = 0; p < G; ++p) {
-        // The opaque index pins this gene's loads here; otherwise the
-        // scheduler lifts every gene's constants to the top of the step.
-        const GrnGene gp = Pq->gene[p + opaque_zero()];
-        const float c = hi[p][k];
-        float tor = gp.tor0, tand = gp.tand0;
+        const GrnGene gp = Pq->gene[p];
+        float tor[CB], tand[CB];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          tor[j] = gp.tor0;
+          tand[j] = gp.tand0;
+        }
+        // one 16-byte scalar load per link: (r, a, b, pad)
+        const float4 *lk = reinterpret_cast<const float4 *>(Pq->link);
         for (int l = gp.link_begin; l < gp.link_begin + gp.n_or; ++l) {
-          const GrnLink L = Pq->link[l];
-          tor = __builtin_fminf(tor, __builtin_fmaf(L.a, lg[L.r], L.b));
+          const float4 L = lk[l];
+          const int r = __float_as_int(L.x);
+#pragma unroll
+          for (int j = 0; j < CB; ++j) tor[j] = __builtin_fminf(tor[j], __builtin_fmaf(L.y, lg[j][r], L.z));
         }
         for (int l = gp.link_begin + gp.n_or; l < gp.link_begin + gp.n_or + gp.n_and; ++l) {
-          const GrnLink L = Pq->link[l];
-          tand = __builtin_fmaxf(tand, __builtin_fmaf(L.a, lg[L.r], L.b));
+          const float4 L = lk[l];
+          const int r = __float_as_int(L.x);
+#pragma unroll
+          for (int j = 0; j < CB; ++j) tand[j] = __builtin_fmaxf(tand[j], __builtin_fmaf(L.y, lg[j][r], L.z));
         }
-        const float h = hw_rcp(1.0f + hw_exp2(__builtin_fmaxf(tor, tand)));
-        float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
-
         const float *pl = sh.lds + p * plane + ubase;
-        const float left = pl[(k + 1) * pc()];
-        const float right = pl[(k + 1) * pc() + 2];
-        const float up = pl[k * pc() + 1];
-        const float down = pl[(k + 2) * pc() + 1];
-        const float S = ((left + right) + up) + down;
-        float lap = __builtin_fmaf(-nf[k], c, S);
-        if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
-          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
-          lap = use ? lap : 0.f;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          const int k = k0 + j;
+          const float c = hi[p][k];
+          const float h = hw_rcp(1.0f + hw_exp2(__builtin_fmaxf(tor[j], tand[j])));
+          float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
+          const float left = pl[(k + 1) * pc()];
+          const float right = pl[(k + 1) * pc() + 2];
+          const float up = pl[k * pc() + 1];
+          const float down = pl[(k + 2) * pc() + 1];
+          const float S = ((left + right) + up) + down;
+          float lap = __builtin_fmaf(-nf[k], c, S);
+          if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
+            const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+            lap = use ? lap : 0.f;
+          }
+          R = __builtin_fmaf(gp.dif, lap, R);
+          // Compensated Euler update c += dt*R (Fast2Sum on the pair) with
+          // the minConc clamp of modelsimulator.cpp:263-275.
+          const float yv = __builtin_fmaf(dt, R, lo[p][k]);
+          const float sv = c + yv;
+          const float nl = yv - (sv - c);
+          const bool clamp = (sv < min_conc) && (clamp_any_r || R != 0.f);
+          hi[p][k] = clamp ? 0.f : sv;
+          lo[p][k] = clamp ? 0.f : nl;
+          if (TRACK) cmax = __builtin_fmaxf(cmax, __builtin_fabsf(R));
         }
-        R = __builtin_fmaf(gp.dif, lap, R);
-
-        // Compensated Euler update c += dt*R (Fast2Sum on the pair) with the
-        // minConc clamp of modelsimulator.cpp:263-275.  The empty asm takes
-        // the results here, so the compiler cannot sink this gene's update
-        // below the next genes' link loops (which keeps every gene's
-        // constants and partial sums alive at once).
-        const float yv = __builtin_fmaf(dt, R, lo[p][k]);
-        const float sv = c + yv;
-        const float nl = yv - (sv - c);
-        const bool clamp = (sv < min_conc) && (clamp_any_r || R != 0.f);
-        hi[p][k] = clamp ? 0.f : sv;
-        lo[p][k] = clamp ? 0.f : nl;
-        asm volatile("" : "+v"(hi[p][k]), "+v"(lo[p][k]));
-        if (TRACK) cmax = __builtin_fmaxf(cmax, __builtin_fabsf(R));
+        // Take this gene's results here so the compiler cannot sink its
+        // update below the next genes' link loops (which would keep every
+        // gene's constants and partial sums alive at once).
+#pragma unroll
+        for (int j = 0; j < CB; ++j) asm volatile("" : "+v"(hi[p][k0 + j]), "+v"(lo[p][k0 + j]));
       }
     }
 
@@ -218,13 +255,13 @@ __device__ __noinline__ double score_terms(const double *stage, const double *ta
 // Simulates and scores one model (all experiments) -- the body of the
 // reference's simulateModelKernel (modelsimulatordevice.cu:216-251) with the
 // CPU semantics of EvaluatorMorphogens::evaluate.
-template <int WC, int G, int CPT, bool GUARD>
+template <int WC, int G, int CPT, int NTMAX, bool GUARD>
 __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *__restrict__ progs,
                                           const GrnDataset *__restrict__ datasets, int m, int pi,
                                           Shared sh) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-  Model<WC, G, CPT, GUARD> f;
+  Model<WC, G, CPT, NTMAX, GUARD> f;
   const GrnProgram *P = progs + pi;
   const GrnDataset *D = datasets + P->dataset;
   f.P = P;
@@ -335,15 +372,15 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
   }
 }
 
-template <int WC, int CPT, bool GUARD, int G, int GHI>
+template <int WC, int CPT, int NTMAX, bool GUARD, int G, int GHI>
 __device__ __forceinline__ void dispatch_g(int g, const GrnLaunch &a, const GrnProgram *__restrict__ progs,
                                            const GrnDataset *__restrict__ datasets, int m, int pi,
                                            Shared sh) {
   if constexpr (G <= GHI) {
     if (g == G)
-      run_model<WC, G, CPT, GUARD>(a, progs, datasets, m, pi, sh);
+      run_model<WC, G, CPT, NTMAX, GUARD>(a, progs, datasets, m, pi, sh);
     else
-      dispatch_g<WC, CPT, GUARD, G + 1, GHI>(g, a, progs, datasets, m, pi, sh);
+      dispatch_g<WC, CPT, NTMAX, GUARD, G + 1, GHI>(g, a, progs, datasets, m, pi, sh);
   }
 }
 
@@ -365,7 +402,7 @@ __global__ __launch_bounds__(NTMAX) void field_kernel(GrnLaunch a, const GrnProg
     if (m >= a.n) break;
     const int pi = __builtin_amdgcn_readfirstlane(order[m]);
     const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
-    dispatch_g<WC, CPT, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
+    dispatch_g<WC, CPT, NTMAX, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
     __syncthreads();
   }
 }
