@@ -73,8 +73,11 @@ struct Shared {
   double *wsum;    // per-wave fitness partial sums
 };
 
-// Everything one model needs; G, CPT, WC compile-time.
-template <int WC, int G, int CPT, int NTMAX, bool GUARD>
+// Everything one model (or one tile of a model) needs; G, CPT, WC
+// compile-time.  TILED: the registers hold an extended tile of a larger grid
+// (grn_tiled.hpp): cells outside the grid get a zero time step so they stay
+// zero, and only the tile's interior cells count towards the max change.
+template <int WC, int G, int CPT, int NTMAX, bool GUARD, bool TILED = false>
 struct Model {
   const GrnProgram *P;
   int W, H, plane;
@@ -83,6 +86,8 @@ struct Model {
   bool active;
   int x, y0, ubase;      // ubase: LDS index of the cell north-west of cell k=0
   float nf[CPT];         // neighbour count per cell
+  float dtc[TILED ? CPT : 1];  // per-cell time step (TILED)
+  float trk[TILED ? CPT : 1];  // 1 where the cell's |R| counts (TILED)
   float hi[G][CPT];
   float lo[G][CPT];
   Shared sh;
@@ -106,7 +111,9 @@ struct Model {
   // of NTMAX threads next to the (hi, lo) state.
   static constexpr int lg_width() { return G <= 4 ? 4 : (G <= 8 ? 8 : 16); }
   static constexpr int budget() { return 131072 / NTMAX < 512 ? 131072 / NTMAX : 512; }
-  static constexpr bool fits(int cb) { return 2 * G * CPT + lg_width() * cb + 4 * cb + 32 <= budget(); }
+  static constexpr bool fits(int cb) {
+    return 2 * G * CPT + (TILED ? 3 * CPT : 0) + lg_width() * cb + 4 * cb + 32 <= budget();
+  }
   static constexpr int cell_block() {
     int best = 1;
     for (int cb = 1; cb <= CPT; ++cb)
@@ -176,13 +183,13 @@ struct Model {
           R = __builtin_fmaf(gp.dif, lap, R);
           // Compensated Euler update c += dt*R (Fast2Sum on the pair) with
           // the minConc clamp of modelsimulator.cpp:263-275.
-          const float yv = __builtin_fmaf(dt, R, lo[p][k]);
+          const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
           const float sv = c + yv;
           const float nl = yv - (sv - c);
           const bool clamp = (sv < min_conc) && (clamp_any_r || R != 0.f);
           hi[p][k] = clamp ? 0.f : sv;
           lo[p][k] = clamp ? 0.f : nl;
-          if (TRACK) cmax = __builtin_fmaxf(cmax, __builtin_fabsf(R));
+          if (TRACK) cmax = __builtin_fmaxf(cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
         }
         // Take this gene's results here so the compiler cannot sink its
         // update below the next genes' link loops (which would keep every

@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "grn.h"
@@ -75,10 +76,19 @@ struct HostDataset {
   void *d_target = nullptr;
 };
 
+struct TileGeom {
+  int tw = 0, th = 0, sx = 0, sy = 0, halo = 0;
+  bool operator==(const TileGeom &o) const {
+    return tw == o.tw && th == o.th && sx == o.sx && sy == o.sy && halo == o.halo;
+  }
+};
+
 struct KernelChoice {
   const GrnKernelEntry *k = nullptr;
   int block = 0;
   size_t lds = 0;
+  TileGeom tile;       // halo-tiled kernels only
+  long slot_floats = 0;  // device-memory state per workgroup (halo-tiled)
 };
 
 struct Ctx {
@@ -91,6 +101,8 @@ struct Ctx {
   DevBuf d_datasets, d_progs, d_order, d_maxerr, d_out, d_counter, d_fields;
   std::vector<const GrnKernelEntry *> kernels;
   std::map<const void *, size_t> lds_set;  // dynamic-LDS attribute already applied
+  DevBuf d_scratch;                         // halo-tiled state slots
+  int tile_halo = 4;                        // Euler steps per tile load
   grn_stats stats{};
 
   ~Ctx() {
@@ -106,7 +118,8 @@ struct Ctx {
 
 void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
   const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w48,
-                                              grn_field_table_w64, grn_field_table_wany};
+                                              grn_field_table_w64, grn_field_table_wany,
+                                              grn_field_table_tiled};
   for (auto fn : tables) {
     int n = 0;
     const GrnKernelEntry *t = fn(&n);
@@ -114,12 +127,45 @@ void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
   }
 }
 
-// Picks the on-chip kernel for a grid of W x H with G genes, or returns an
-// error when no instantiation can hold it.
-int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, KernelChoice *out) {
+// Window of the halo-tiled kernel for a W x H grid: extended tiles of
+// tw x th cells (th a multiple of cpt, at most ntmax threads, G LDS planes
+// within the LDS budget), interior (tw - 2 halo) x (th - 2 halo).  Minimises
+// the cells computed per step over all tiles plus a per-tile overhead.
+bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int halo, TileGeom *out) {
+  double best = 1e300;
+  bool found = false;
+  const int th_max = ((H + 2 * halo + cpt - 1) / cpt) * cpt;
+  for (int th = cpt; th <= th_max; th += cpt) {
+    const int sy = std::min(th - 2 * halo, H);
+    if (sy < 1) continue;
+    for (int tw = 2 * halo + 1; tw <= W + 2 * halo; ++tw) {
+      const int sx = std::min(tw - 2 * halo, W);
+      const long threads = (long)tw * (th / cpt);
+      if (threads > ntmax) break;
+      const size_t lds = (size_t)G * (tw + 2) * (th + 2) * sizeof(float);
+      if (lds > max_lds) break;
+      const int ntx = (W + sx - 1) / sx, nty = (H + sy - 1) / sy;
+      const double cost = (double)ntx * nty * ((double)tw * th + 2048.0);
+      if (cost < best) {
+        best = cost;
+        found = true;
+        out->tw = tw;
+        out->th = th;
+        out->sx = sx;
+        out->sy = sy;
+        out->halo = halo;
+      }
+    }
+  }
+  return found;
+}
+
+// Picks the kernel for a grid of W x H with G genes: an on-chip kernel when
+// the field fits (unless the tiled path is forced), else the halo-tiled one.
+int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path, KernelChoice *out) {
   const KernelChoice none;
   *out = none;
-  for (int pass = 0; pass < 2; ++pass) {  // exact-width kernels first, then any-width
+  for (int pass = 0; pass < 2 && path != GRN_PATH_TILED; ++pass) {  // exact width first, then any width
     for (const GrnKernelEntry *k : c.kernels) {
       if (pass == 0 ? k->wc != W : k->wc != 0) continue;
       if (G < k->glo || G > k->ghi || k->guard != guard) continue;
@@ -134,6 +180,19 @@ int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, Kern
       out->k = k;
       out->block = block;
       out->lds = lds;
+      return 0;
+    }
+  }
+  if (path != GRN_PATH_LDS) {
+    for (const GrnKernelEntry *k : c.kernels) {
+      if (k->wc != -1 || G < k->glo || G > k->ghi || k->guard != guard) continue;
+      TileGeom t;
+      if (!tile_geometry(W, H, G, k->cpt, k->ntmax, c.max_lds, c.tile_halo, &t)) continue;
+      out->k = k;
+      out->tile = t;
+      out->block = (int)(((long)t.tw * (t.th / k->cpt) + 63) / 64 * 64);
+      out->lds = (size_t)G * (t.tw + 2) * (t.th + 2) * sizeof(float);
+      out->slot_floats = 4L * G * W * H;
       return 0;
     }
   }
@@ -173,7 +232,7 @@ struct Item {
 // Compiles, uploads, launches and collects one batch.  fields (optional)
 // receives each model's final field of experiment field_exp.
 int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
-              const double *max_errors, const double *dt_override, grn_result *out,
+              const double *max_errors, const double *dt_override, int path, grn_result *out,
               double *fields_host, long field_stride, int field_exp, int *n_products_out) {
   const auto t_start = std::chrono::steady_clock::now();
   if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
@@ -191,7 +250,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     rc = grn::compile_model(m, n_morph, ds, (float)dt, &items[i].prog, &items[i].info, &err);
     if (rc) return fail(rc, "model %d: %s", i, err.c_str());
     const int W = D.params.dim_x, H = D.params.dim_y;
-    rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, &items[i].kc);
+    rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, path, &items[i].kc);
     if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
     items[i].dataset = ds;
     items[i].cost = (double)W * H * D.params.num_sim_steps * D.n_exp * items[i].info.flops_per_cell_step;
@@ -202,8 +261,24 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   // time order for the persistent work queue).
   std::vector<int> idx(n);
   for (int i = 0; i < n; ++i) idx[i] = i;
+  auto same_class = [&](int a, int b) {
+    const KernelChoice &x = items[a].kc, &y = items[b].kc;
+    if (x.k != y.k || !(x.tile == y.tile)) return false;
+    if (x.k->wc == -1) {  // the tiled window is per grid size
+      const grn_sim_params &pa = c->datasets[items[a].dataset].params, &pb = c->datasets[items[b].dataset].params;
+      return pa.dim_x == pb.dim_x && pa.dim_y == pb.dim_y;
+    }
+    return true;
+  };
+  auto class_key = [&](int a) {
+    const KernelChoice &x = items[a].kc;
+    const grn_sim_params &pa = c->datasets[items[a].dataset].params;
+    return std::make_tuple((const void *)x.k, x.tile.tw, x.tile.th, x.tile.sx, x.tile.sy,
+                           x.k->wc == -1 ? pa.dim_x : 0, x.k->wc == -1 ? pa.dim_y : 0);
+  };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-    if (items[a].kc.k != items[b].kc.k) return items[a].kc.k < items[b].kc.k;
+    const auto ka = class_key(a), kb = class_key(b);
+    if (ka != kb) return ka < kb;
     return items[a].cost > items[b].cost;
   });
 
@@ -222,7 +297,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   if (fields_host) {
     if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
     for (int i = 1; i < n; ++i)
-      if (items[i].kc.k != items[0].kc.k) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
+      if (!same_class(i, 0)) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
   }
   hipStream_t st = c->stream;
   if (n > 0) {
@@ -236,7 +311,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
   for (int i = 0; i < n;) {
     int j = i;
-    while (j < n && items[idx[j]].kc.k == items[idx[i]].kc.k) ++j;
+    while (j < n && same_class(idx[j], idx[i])) ++j;
     groups.push_back({i, j});
     i = j;
   }
@@ -251,6 +326,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     for (int j = b; j < e; ++j) {
       kc.block = std::max(kc.block, items[idx[j]].kc.block);
       kc.lds = std::max(kc.lds, items[idx[j]].kc.lds);
+      kc.slot_floats = std::max(kc.slot_floats, items[idx[j]].kc.slot_floats);
     }
     auto it = c->lds_set.find(kc.k->fn);
     if (it == c->lds_set.end() || it->second < kc.lds) {
@@ -261,7 +337,18 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
     if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
     const int grid = std::min(e - b, per_cu * c->n_cu);
+    if (kc.slot_floats > 0) {
+      int rc2 = c->d_scratch.reserve(sizeof(float) * (size_t)kc.slot_floats * grid);
+      if (rc2) return rc2;
+    }
     GrnLaunch L{};
+    L.scratch = (float *)c->d_scratch.p;
+    L.slot_floats = kc.slot_floats;
+    L.tile_w = kc.tile.tw;
+    L.tile_h = kc.tile.th;
+    L.tile_sx = kc.tile.sx;
+    L.tile_sy = kc.tile.sy;
+    L.tile_halo = kc.tile.halo;
     L.progs = (const GrnProgram *)c->d_progs.p;
     L.order = (const int *)c->d_order.p + b;
     L.n = e - b;
@@ -447,23 +534,23 @@ int grn_evaluate(void *ctx, int dataset_id, const grn_model *model, double max_e
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(GRN_E_INVALID, "null context");
   HIP_TRY(hipSetDevice(c->device));
-  return run_batch(c, 1, &dataset_id, model, &max_error, nullptr, out, nullptr, 0, 0, nullptr);
+  return run_batch(c, 1, &dataset_id, model, &max_error, nullptr, GRN_PATH_AUTO, out, nullptr, 0, 0, nullptr);
 }
 
 int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
                        const double *max_errors, const double *dt_override, int path, grn_result *out) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(GRN_E_INVALID, "null context");
-  if (path == GRN_PATH_TILED) return fail(GRN_E_CAPACITY, "the halo-tiled path is not built");
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
   HIP_TRY(hipSetDevice(c->device));
-  return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, out, nullptr, 0, 0, nullptr);
+  return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, path, out, nullptr, 0, 0, nullptr);
 }
 
 int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, double dt_override, int experiment,
                         int path, double *fields, long capacity, int *n_products) {
   Ctx *c = as_ctx(ctx);
   if (!c || !fields) return fail(GRN_E_INVALID, "null argument");
-  if (path == GRN_PATH_TILED) return fail(GRN_E_CAPACITY, "the halo-tiled path is not built");
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
   if (dataset_id < 0 || dataset_id >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "unknown dataset");
   const grn_sim_params &p = c->datasets[dataset_id].params;
   if (experiment < 0 || experiment >= c->datasets[dataset_id].n_exp) return fail(GRN_E_INVALID, "bad experiment");
@@ -474,7 +561,7 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, doubl
   int g = 0;
   const double dt = dt_override;
   const double inf = std::numeric_limits<double>::infinity();
-  int rc = run_batch(c, 1, &dataset_id, model, &inf, &dt, &r, buf.data(), stride, experiment, &g);
+  int rc = run_batch(c, 1, &dataset_id, model, &inf, &dt, path, &r, buf.data(), stride, experiment, &g);
   if (rc) return rc;
   const long need = (long)g * p.dim_x * p.dim_y;
   if (capacity < need) return fail(GRN_E_INVALID, "fields buffer holds %ld of %ld values", capacity, need);

@@ -3,7 +3,7 @@
 #define GRN_KERNELS_H
 
 struct GrnKernelEntry {
-  int wc;     // compile-time grid width, 0 = any width
+  int wc;     // compile-time grid width, 0 = any width, -1 = halo-tiled kernel
   int cpt;    // cells per thread (vertical strip length)
   int ntmax;  // workgroup size bound
   int glo;    // smallest gene count handled
@@ -17,5 +17,6 @@ extern "C" const GrnKernelEntry *grn_field_table_w32(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 
 #endif

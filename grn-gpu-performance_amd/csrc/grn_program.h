@@ -86,6 +86,13 @@ struct GrnLaunch {
   long field_stride;        // elements per model in fields
   int field_exp;            // experiment whose final field is written
   int pad;
+  // halo-tiled kernel only: device-memory state and tile geometry
+  float *scratch;           // per workgroup: 4 planes sets (hi, lo) x (A, B)
+  long slot_floats;         // floats per workgroup slot
+  int tile_sx, tile_sy;     // interior of a tile (cells written back)
+  int tile_halo;            // halo width = Euler steps per tile load
+  int tile_w, tile_h;       // extended tile (interior + halos, padded)
+  int pad2;
 };
 
 #endif

@@ -1,0 +1,227 @@
+// grn_tiled.hpp -- halo-tiled kernel for grids whose field does not fit the
+// LDS of one workgroup (e.g. 96x96, 100x100, 128x128 with many genes).
+//
+// One workgroup still owns one model at a time (persistent, work queue).  The
+// model's state lives in device memory as two (hi, lo) plane sets, A (time s)
+// and B (time s + T).  The grid is covered by tiles of tile_sx x tile_sy
+// cells; each tile is processed in an extended window of tile_w x tile_h cells
+// (a halo of tile_halo = T cells on every side):
+//   load the window from A into registers and the LDS mirror, run T Euler
+//   steps with the on-chip step code of grn_field.hpp, write the interior
+//   back to B.
+// A window cell at distance d from the window edge is exact for d steps, so
+// after T steps the interior (d >= T) is exact; the halo is recomputed by the
+// neighbouring tiles (temporal blocking: one device-memory round trip per T
+// steps instead of per step).  Cells outside the grid are held at zero with a
+// zero time step and the no-flux neighbour count comes from global
+// coordinates.  With an early-stop threshold (minConcChange >= 0) every step
+// needs the grid-wide max change, so T drops to one step per round trip.
+#pragma once
+#include "grn_field.hpp"
+
+namespace grn_field {
+
+// Fitness terms straight from the device-memory state (c = hi + lo).
+__device__ __noinline__ double score_terms_split(const float *hi, const float *lo, const double *target,
+                                                 int count, double thr, int mse) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const double d = ((double)hi[i] + (double)lo[i]) - target[i];
+    if (mse) {
+      acc += d * d;
+    } else {
+      const double u = __builtin_fabs(d) - thr;
+      if (u > 0) acc += log(1.0 + u);
+    }
+  }
+  return acc;
+}
+
+template <int G, int CPT, int NTMAX, bool GUARD>
+__device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnProgram *__restrict__ progs,
+                                                const GrnDataset *__restrict__ datasets, int m, int pi,
+                                                Shared sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const GrnProgram *P = progs + pi;
+  const GrnDataset *D = datasets + P->dataset;
+  const int W = D->W, H = D->H, N = W * H;
+  const int TW = a.tile_w, TH = a.tile_h, T = a.tile_halo, SX = a.tile_sx, SY = a.tile_sy;
+  const int ntx = (W + SX - 1) / SX, nty = (H + SY - 1) / SY;
+  float *slot = a.scratch + (size_t)blockIdx.x * a.slot_floats;
+  float *Ahi = slot, *Alo = slot + (size_t)G * N, *Bhi = slot + (size_t)2 * G * N, *Blo = slot + (size_t)3 * G * N;
+
+  Model<0, G, CPT, NTMAX, GUARD, true> f;
+  f.P = P;
+  f.sh = sh;
+  f.W = TW;
+  f.H = TH;
+  f.plane = (TW + 2) * (TH + 2);
+  f.dt = P->dt;
+  f.min_conc = D->min_conc;
+  f.clamp_any_r = !(D->min_conc > 0.f || !D->nonneg);
+  const int n_active = TW * (TH / CPT);  // the launcher guarantees tile_h % CPT == 0
+  f.active = tid < n_active;
+  f.x = f.active ? tid % TW : 0;
+  f.y0 = f.active ? (tid / TW) * CPT : 0;
+  f.ubase = f.y0 * (TW + 2) + f.x;
+  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt, S = D->steps;
+  const float mcc = D->min_conc_change;
+  const bool early = mcc >= 0.f;
+  const double max_err = a.max_err[pi];
+  const int plane_floats = G * f.plane;
+
+  double err = 0.0, chg_acc = 0.0;
+  int e = 0, steps_total = 0;
+  while (e < n_exp && err <= max_err) {
+    // Initial state at time 0 in A: inputs for genes < nIn, zero elsewhere.
+    for (int i = tid; i < G * N; i += blockDim.x) {
+      const int g = i / N;
+      const double v = g < n_in ? D->init[(size_t)e * n_in * N + i] : 0.0;
+      const float h = (float)v;
+      Ahi[i] = h;
+      Alo[i] = (float)(v - (double)h);
+    }
+    __syncthreads();
+
+    int s = 0;
+    float chg = 1.0f;
+    while (s < S && (!early || chg > mcc)) {
+      const int Tb = early ? 1 : (S - s < T ? S - s : T);
+      const bool track = early || s + Tb == S;
+      float cblk = 0.f;
+      for (int ty = 0; ty < nty; ++ty) {
+        for (int tx = 0; tx < ntx; ++tx) {
+          const int ox = tx * SX - T, oy = ty * SY - T;
+          const int gx = ox + f.x;
+          int gidx[CPT];
+#pragma unroll
+          for (int k = 0; k < CPT; ++k) {
+            const int yt = f.y0 + k, gy = oy + yt;
+            const bool in = f.active && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const bool interior = in && f.x >= T && f.x < T + SX && yt >= T && yt < T + SY;
+            f.nf[k] = in ? (float)((gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1)) : 0.f;
+            f.dtc[k] = in ? f.dt : 0.f;
+            f.trk[k] = interior ? 1.f : 0.f;
+            gidx[k] = in ? gy * W + gx : -1;
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+              // load from an in-range index, then select: no out-of-range
+              // address is ever formed for cells outside the grid
+              const size_t i = (size_t)g * N + (gidx[k] >= 0 ? gidx[k] : 0);
+              const float vh = Ahi[i], vl = Alo[i];
+              f.hi[g][k] = gidx[k] >= 0 ? vh : 0.f;
+              f.lo[g][k] = gidx[k] >= 0 ? vl : 0.f;
+            }
+          for (int i = tid; i < plane_floats; i += blockDim.x) sh.lds[i] = 0.f;
+          __syncthreads();
+          f.store_planes();
+          __syncthreads();
+          float c = 0.f;
+          for (int t = 0; t < Tb; ++t) {
+            if (track && t == Tb - 1)
+              c = f.template step<true>();
+            else
+              f.template step<false>();
+          }
+          cblk = __builtin_fmaxf(cblk, c);
+#pragma unroll
+          for (int k = 0; k < CPT; ++k)
+            if (f.trk[k] != 0.f)
+#pragma unroll
+              for (int g = 0; g < G; ++g) {
+                Bhi[(size_t)g * N + gidx[k]] = f.hi[g][k];
+                Blo[(size_t)g * N + gidx[k]] = f.lo[g][k];
+              }
+        }
+      }
+      float *t0 = Ahi, *t1 = Alo;
+      Ahi = Bhi;
+      Alo = Blo;
+      Bhi = t0;
+      Blo = t1;
+      __syncthreads();  // the new state is complete before the next block reads it
+      s += Tb;
+      if (track) chg = cblk;
+    }
+    steps_total += s;
+    chg_acc += (double)chg;
+
+    const bool mse = D->kind == 1;
+    double acc = score_terms_split(Ahi + (size_t)n_in * N, Alo + (size_t)n_in * N,
+                                   D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr, mse ? 1 : 0);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) sh.wsum[wave] = acc;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < nw; ++w) tot += sh.wsum[w];
+    double e_err;
+    if (mse) {
+      e_err = tot / ((double)N * n_tgt);
+    } else {
+      e_err = tot / N;
+      if (chg_acc > (double)mcc) e_err += chg_acc - (double)mcc;  // evaluatormorphogens.cpp:62-63
+    }
+    err += e_err / n_exp;
+    if (a.fields && e == a.field_exp) {
+      double *fo = a.fields + (size_t)m * a.field_stride;
+      for (int i = tid; i < G * N; i += blockDim.x) fo[i] = (double)Ahi[i] + (double)Alo[i];
+    }
+    ++e;
+    __syncthreads();  // wsum reads and the state reads done before reuse
+  }
+
+  if (tid == 0) {
+    GrnModelOut o;
+    o.err = err;
+    o.change = chg_acc;
+    o.steps = steps_total;
+    o.exps = e;
+    o.done = 1;
+    o.pad = 0;
+    a.out[pi] = o;
+  }
+}
+
+template <int CPT, int NTMAX, bool GUARD, int G, int GHI>
+__device__ __forceinline__ void dispatch_tiled(int g, const GrnLaunch &a, const GrnProgram *__restrict__ progs,
+                                               const GrnDataset *__restrict__ datasets, int m, int pi,
+                                               Shared sh) {
+  if constexpr (G <= GHI) {
+    if (g == G)
+      run_tiled_model<G, CPT, NTMAX, GUARD>(a, progs, datasets, m, pi, sh);
+    else
+      dispatch_tiled<CPT, NTMAX, GUARD, G + 1, GHI>(g, a, progs, datasets, m, pi, sh);
+  }
+}
+
+template <int CPT, int NTMAX, int GLO, int GHI, bool GUARD>
+__global__ __launch_bounds__(NTMAX) void tiled_kernel(GrnLaunch a, const GrnProgram *__restrict__ progs,
+                                                      const GrnDataset *__restrict__ datasets,
+                                                      const int *__restrict__ order) {
+  extern __shared__ float lds[];
+  __shared__ int s_model;
+  __shared__ float s_wmax[NTMAX / 64];
+  __shared__ double s_wsum[NTMAX / 64];
+  Shared sh{lds, s_wmax, s_wsum};
+  for (;;) {
+    if (threadIdx.x == 0) s_model = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int m = __builtin_amdgcn_readfirstlane(s_model);
+    __syncthreads();
+    if (m >= a.n) break;
+    const int pi = __builtin_amdgcn_readfirstlane(order[m]);
+    const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
+    dispatch_tiled<CPT, NTMAX, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
+    __syncthreads();
+  }
+}
+
+}  // namespace grn_field
+
+#define GRN_TILED_ENTRY(CPT, NTMAX, GLO, GHI, GUARD) \
+  {-1, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::tiled_kernel<CPT, NTMAX, GLO, GHI, GUARD>}

@@ -209,3 +209,58 @@ def test_config1_parity_sample(ctx):
     workloads.make_target(ctx, w)
     ds, got, want = run_both(ctx, w.params, w.init, w.target, w.models)
     check_fitness(got["raw_error"], want["raw_error"])
+
+
+# ---- halo-tiled path (grids beyond one workgroup's LDS) ------------------------
+
+@pytest.mark.parametrize("W,H,gr", [(96, 96, (3, 8)), (100, 100, (4, 10)), (128, 128, (4, 12)),
+                                    (130, 70, (3, 16))])
+def test_tiled_path_parity(ctx, W, H, gr):
+    p, init, target, models = population_case(31 + W + H, 10, W, H, 120, n_in=3, g_range=gr,
+                                              l_range=(1, 40))
+    ds, got, want = run_both(ctx, p, init, target, models)
+    assert np.array_equal(got["n_products"], want["n_products"])
+    check_fitness(got["raw_error"], want["raw_error"])
+    f = ctx.simulate_fields(models[0], dataset=ds)
+    _, fo = ora.evaluate(ora.from_simparams(p), init, target, models[0].genes, models[0].links,
+                         want_fields=True)
+    check_fields(f, fo)
+
+
+@pytest.mark.parametrize("W,H", [(64, 64), (48, 48), (32, 32)])
+def test_tiled_equals_on_chip_fields(ctx, W, H):
+    # the same step code on the same values: the field is bit-identical, the
+    # fitness differs only by the order of its final FP64 sum
+    p, init, target, models = population_case(41 + W, 6, W, H, 77, n_in=3, g_range=(4, 8),
+                                              l_range=(2, 20))
+    ds = ctx.add_dataset(p, init, target)
+    for m in models:
+        a = ctx.simulate_fields(m, dataset=ds, path=grn.PATH_LDS)
+        b = ctx.simulate_fields(m, dataset=ds, path=grn.PATH_TILED)
+        assert np.array_equal(a, b)
+    ra = ctx.evaluate_batch(models, datasets=ds, path=grn.PATH_LDS)
+    rb = ctx.evaluate_batch(models, datasets=ds, path=grn.PATH_TILED)
+    np.testing.assert_allclose(rb["raw_error"], ra["raw_error"], rtol=1e-12)
+
+
+def test_tiled_early_stop_and_guard(ctx):
+    W, H = 100, 100
+    p, init, target, models = population_case(51, 8, W, H, 1500, g_range=(3, 6), l_range=(1, 10),
+                                              kind=grn.FITNESS_REF, mcc=2e-3, thr=0.05, dt=0.05)
+    ds, got, want = run_both(ctx, p, init, target, models)
+    assert np.mean(got["steps"] == want["steps"]) >= 0.75
+    same = got["steps"] == want["steps"]
+    pen_w = np.where(want["change"] > p.min_conc_change, want["change"] - p.min_conc_change, 0.0)
+    pen_g = np.where(got["change"] > p.min_conc_change, got["change"] - p.min_conc_change, 0.0)
+    check_fitness((got["raw_error"] - pen_g)[same], (want["raw_error"] - pen_w)[same])
+    # negative inputs route to the guarded tiled kernel
+    rng = np.random.default_rng(3)
+    q = grn.SimParams(W, H, 2, 1, 100, 0.02, min_conc=-10.0, fitness_kind=grn.FITNESS_MSE)
+    init2 = rng.uniform(-0.5, 1, (1, 2, H, W))
+    tgt2 = rng.uniform(0, 1, (1, 1, H, W))
+    g = np.zeros(3, grn.GENE_DTYPE)
+    g["label"] = [0, 1, 2]
+    g["lim"], g["deg"], g["dif"] = 0.5, 0.2, 0.1
+    m = grn.Model(g, np.zeros(0, grn.LINK_DTYPE))
+    ds2, got2, want2 = run_both(ctx, q, init2, tgt2, [m])
+    check_fitness(got2["raw_error"], want2["raw_error"])
