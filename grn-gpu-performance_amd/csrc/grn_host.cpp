@@ -425,6 +425,20 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
 
 }  // namespace
 
+namespace grn {
+
+int set_last_error(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+}  // namespace grn
+
 extern "C" {
 
 int grn_abi_version(void) { return GRN_ABI_VERSION; }

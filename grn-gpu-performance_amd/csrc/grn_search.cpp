@@ -1,0 +1,662 @@
+// grn_search.cpp -- island search by deterministic crowding (the caller of
+// the simulation path); see grn_search.hpp for the reference map.
+#include "grn_search.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <set>
+#include <string>
+
+namespace grn {
+
+// ---- RNG ---------------------------------------------------------------------
+
+static uint64_t splitmix64(uint64_t *x) {
+  uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+Rng::Rng(uint64_t seed) {
+  uint64_t x = seed;
+  for (uint64_t &w : s_) w = splitmix64(&x);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+uint64_t Rng::next() {
+  const uint64_t r = rotl(s_[1] * 5, 7) * 9;
+  const uint64_t t = s_[1] << 17;
+  s_[2] ^= s_[0];
+  s_[3] ^= s_[1];
+  s_[1] ^= s_[2];
+  s_[0] ^= s_[3];
+  s_[2] ^= t;
+  s_[3] = rotl(s_[3], 45);
+  return r;
+}
+
+double Rng::uniform01() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+double Rng::uniform(double lo, double hi) { return lo + (hi - lo) * uniform01(); }
+int Rng::below(int n) { return (int)(((unsigned __int128)next() * (unsigned)n) >> 64); }
+
+// ---- operators on models -----------------------------------------------------
+
+static bool has_label(const std::set<int> &s, int l) { return s.count(l) != 0; }
+
+static std::set<int> labels_of(const Model &m) {
+  std::set<int> s;
+  for (const Gene &g : m.genes) s.insert(g.label);
+  return s;
+}
+
+static int find_link(const Model &m, int regulator, int regulated) {
+  for (size_t i = 0; i < m.links.size(); ++i)
+    if (m.links[i].regulator == regulator && m.links[i].regulated == regulated) return (int)i;
+  return -1;
+}
+
+// Model::removeLink(regulator, regulated): the first matching link, if any.
+static void remove_link(Model *m, int regulator, int regulated) {
+  const int i = find_link(*m, regulator, regulated);
+  if (i >= 0) m->links.erase(m->links.begin() + i);
+}
+
+// Model::removeProduct / removeProductWithLabel: the product and every link
+// touching its label.
+static void remove_product_at(Model *m, int i) {
+  const int label = m->genes[i].label;
+  m->genes.erase(m->genes.begin() + i);
+  for (int j = (int)m->links.size() - 1; j >= 0; --j)
+    if (m->links[j].regulator == label || m->links[j].regulated == label) m->links.erase(m->links.begin() + j);
+}
+
+Gene Search::random_gene(Rng &rng, int label) {
+  // ModelProd(nMorphogens, label) (modelprod.cpp:14-24); zeroInit tags the
+  // red/green/blue morphogens with -1/-2/-3 (modelprod.cpp:61-70)
+  Gene g;
+  g.label = label;
+  g.init = (label < 3 && label < n_morph_) ? -(double)(label + 1) : 0.0;
+  g.lim = rng.uniform(p_.lim_lo, p_.lim_hi);
+  g.deg = rng.uniform(p_.deg_lo, p_.deg_hi);
+  g.dif = rng.uniform01() < p_.dif_prob ? rng.uniform(p_.dif_lo, p_.dif_hi) : 0.0;
+  return g;
+}
+
+Link Search::random_link(Rng &rng, int regulator, int regulated) {
+  // ModelLink(rtor, rted, isActivation = false) (modellink.cpp:10-22)
+  Link l;
+  l.regulator = regulator;
+  l.regulated = regulated;
+  l.dis_const = rng.uniform(p_.k_lo, p_.k_hi);
+  const double n = rng.uniform(p_.n_lo, p_.n_hi);
+  l.hill_coef = rng.coin() ? n : -n;
+  l.is_and = l.hill_coef < 0 || rng.coin();
+  return l;
+}
+
+void Search::mutate_gene_params(Rng &rng, Gene *g) {
+  // ModelProd::mutateParams(1, nMorphogens) (modelprod.cpp:43-59)
+  if (rng.below(100) < 1) g->lim = rng.uniform(p_.lim_lo, p_.lim_hi);
+  if (rng.below(100) < 1) g->deg = rng.uniform(p_.deg_lo, p_.deg_hi);
+  if (rng.below(100) < 1) g->dif = rng.coin() ? rng.uniform(p_.dif_lo, p_.dif_hi) : 0.0;
+}
+
+void Search::mutate_link_params(Rng &rng, Link *l) {
+  // ModelLink::mutateParams(1) (modellink.cpp:24-40)
+  if (rng.below(100) < 1) l->dis_const = rng.uniform(p_.k_lo, p_.k_hi);
+  if (rng.below(100) < 1) {
+    const double n = rng.uniform(p_.n_lo, p_.n_hi);
+    l->hill_coef = rng.coin() ? n : -n;
+  }
+  if (l->hill_coef < 0)
+    l->is_and = true;
+  else if (rng.below(100) < 1)
+    l->is_and = rng.coin();
+}
+
+Model Search::create_random(Rng &rng, int n_products) {
+  // Model::createRandom (model.cpp:285-315)
+  Model m;
+  for (int i = 0; i < n_products; ++i) m.genes.push_back(random_gene(rng, i));
+  for (int i = 0; i < n_products; ++i) {
+    const int n_reg = rng.below(1 + n_products);
+    for (int j = 0; j < n_reg; ++j) {
+      const int regulated = rng.below(n_products);
+      remove_link(&m, i, regulated);  // addOrReplaceRandomLink (model.cpp:449-453)
+      m.links.push_back(random_link(rng, i, regulated));
+    }
+  }
+  return m;
+}
+
+void Search::remove_excess(Rng &rng, Model *m) {
+  // Model::removeExcess(nMinProducts, nMaxProducts, nMaxLinks) (model.cpp:550-574)
+  const int n_min = n_morph_;
+  if (p_.n_max_products > -1) {
+    std::vector<int> in_use = m->labels_in_use(n_min);
+    int to_remove = (int)in_use.size() - p_.n_max_products;
+    if (to_remove > 0) {
+      in_use.erase(std::remove_if(in_use.begin(), in_use.end(), [&](int l) { return l < n_min; }), in_use.end());
+      for (int i = 0; i < to_remove && !in_use.empty(); ++i) {
+        const int k = rng.below((int)in_use.size());
+        const int label = in_use[k];
+        for (size_t g = 0; g < m->genes.size(); ++g)
+          if (m->genes[g].label == label) {
+            remove_product_at(m, (int)g);
+            break;
+          }
+        in_use.erase(in_use.begin() + k);
+      }
+    }
+  }
+  if (p_.n_max_links > -1) {
+    // calcLinksInUse: links whose regulator and regulated are both in use
+    const std::vector<int> in_use = m->labels_in_use(n_min);
+    const std::set<int> s(in_use.begin(), in_use.end());
+    std::vector<int> links_in_use;
+    for (size_t i = 0; i < m->links.size(); ++i)
+      if (s.count(m->links[i].regulator) && s.count(m->links[i].regulated)) links_in_use.push_back((int)i);
+    int to_remove = (int)links_in_use.size() - p_.n_max_links;
+    std::vector<bool> drop(m->links.size(), false);
+    for (int i = 0; i < to_remove; ++i) {
+      const int k = rng.below((int)links_in_use.size());
+      drop[links_in_use[k]] = true;
+      links_in_use.erase(links_in_use.begin() + k);
+    }
+    std::vector<Link> kept;
+    for (size_t i = 0; i < m->links.size(); ++i)
+      if (!drop[i]) kept.push_back(m->links[i]);
+    m->links.swap(kept);
+  }
+}
+
+void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model *c2) {
+  // Model::cross (model.cpp:21-85): product-based uniform crossover
+  const std::set<int> prods1 = labels_of(p1), prods2 = labels_of(p2);
+  std::set<int> common, only1, only2;
+  for (int l : prods1) (has_label(prods2, l) ? common : only1).insert(l);
+  for (int l : prods2)
+    if (!has_label(prods1, l)) only2.insert(l);
+  // distributeProducts: each label goes to the second set with probability 1/2
+  auto distribute = [&](const std::set<int> &from, std::set<int> *to1, std::set<int> *to2) {
+    for (int l : from) (rng.below(100) < 50 ? to2 : to1)->insert(l);
+  };
+  std::set<int> copied, swapped, ex1to1, ex1to2, ex2to2, ex2to1;
+  distribute(common, &copied, &swapped);
+  distribute(only1, &ex1to1, &ex1to2);
+  distribute(only2, &ex2to2, &ex2to1);
+  auto unite = [](const std::set<int> &a, const std::set<int> &b) {
+    std::set<int> u = a;
+    u.insert(b.begin(), b.end());
+    return u;
+  };
+  const int n_morph = n_morph_;
+  // copyProductsAndLinks (model.cpp:101-184)
+  auto build = [&](const Model &from1, const std::set<int> &products1, const Model &from2,
+                   const std::set<int> &products2) {
+    Model to;
+    for (const Gene &g : from1.genes)
+      if (has_label(products1, g.label)) to.genes.push_back(g);
+    for (const Gene &g : from2.genes)
+      if (has_label(products2, g.label)) to.genes.push_back(g);
+    // candidate substitute regulators: the other side's non-morphogen labels
+    std::vector<int> regs1, regs2;
+    for (int l : products1)
+      if (l >= n_morph) regs1.push_back(l);
+    for (int l : products2)
+      if (l >= n_morph) regs2.push_back(l);
+    auto copy_side = [&](const Model &from, const std::set<int> &mine, const std::set<int> &other,
+                         const std::vector<int> &subst) {
+      for (const Link &l : from.links) {
+        if (!has_label(mine, l.regulated)) continue;
+        if (has_label(mine, l.regulator) || has_label(other, l.regulator)) {
+          to.links.push_back(l);
+        } else if (!subst.empty()) {
+          const int r = subst[rng.below((int)subst.size())];
+          if (find_link(from, r, l.regulated) < 0) {
+            Link nl = l;
+            nl.regulator = r;
+            to.links.push_back(nl);
+          }
+        }
+      }
+    };
+    copy_side(from1, products1, products2, regs2);
+    copy_side(from2, products2, products1, regs1);
+    return to;
+  };
+  *c1 = build(p1, unite(copied, ex1to1), p2, unite(swapped, ex2to1));
+  *c2 = build(p2, unite(copied, ex2to2), p1, unite(swapped, ex1to2));
+  if (copied.size() + ex1to1.size() < swapped.size() + ex2to1.size()) std::swap(*c1, *c2);
+  remove_excess(rng, c1);
+  remove_excess(rng, c2);
+}
+
+void Search::mutate(Rng &rng, Model *m) {
+  // Model::mutate (model.cpp:499-547)
+  const int n_min = n_morph_;
+  for (int i = (int)m->genes.size() - 1; i >= 0; --i) {
+    const int label = m->genes[i].label;
+    if (rng.below(100) < 1) {
+      // duplicateProduct (model.cpp:317-333)
+      Gene g = m->genes[i];
+      g.init = 0;
+      g.label = new_label();
+      m->genes.push_back(g);
+      const int regulated = m->genes[rng.below((int)m->genes.size())].label;
+      remove_link(m, g.label, regulated);
+      m->links.push_back(random_link(rng, g.label, regulated));
+      std::vector<int> labels;
+      for (const Gene &x : m->genes) labels.push_back(x.label);
+      std::sort(labels.begin(), labels.end());
+      const int span = (int)labels.size() - n_min;
+      if (span > 0) {
+        const int regulator = labels[n_min + rng.below(span)];
+        remove_link(m, regulator, g.label);
+        m->links.push_back(random_link(rng, regulator, g.label));
+      }
+    }
+    if (label >= n_min && rng.below(1000) < 15) remove_product_at(m, i);
+  }
+  for (int i = (int)m->links.size() - 1; i >= 0; --i) {
+    if (rng.below(100) < 1 && !m->genes.empty()) {
+      // duplicateLink (model.cpp:472-483)
+      Link nl = m->links[i];
+      const int regulator = m->genes[rng.below((int)m->genes.size())].label;
+      const int regulated = m->genes[rng.below((int)m->genes.size())].label;
+      remove_link(m, regulator, regulated);
+      nl.regulator = regulator;
+      nl.regulated = regulated;
+      m->links.push_back(nl);
+    }
+    if (rng.below(1000) < 15 && i < (int)m->links.size()) m->links.erase(m->links.begin() + i);
+  }
+  remove_excess(rng, m);
+  for (Gene &g : m->genes) mutate_gene_params(rng, &g);
+  for (Link &l : m->links) mutate_link_params(rng, &l);
+}
+
+// ---- the search ---------------------------------------------------------------
+
+Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
+  if (p_.island_size % 2) p_.island_size++;  // searchalgodetcrowd.cpp:33-35
+  n_morph_ = p_.n_morphogens;
+  for (int i = 0; i <= p_.n_islands; ++i) rngs_.emplace_back(p_.seed * 0x100000001B3ULL + 0x9E37u * (uint64_t)(i + 1));
+  islands_.resize(p_.n_islands);
+  children_.resize(p_.n_islands);
+  perm_.resize(p_.n_islands);
+  for (int is = 0; is < p_.n_islands; ++is) {
+    Rng &rng = rngs_[is];
+    perm_[is].resize(p_.island_size);
+    std::iota(perm_[is].begin(), perm_[is].end(), 0);
+    for (int k = 0; k < p_.island_size; ++k) {
+      // SearchAlgoDetCrowd::calcInitialPopulation (searchalgodetcrowd.cpp:54-61)
+      const int lo = std::max(p_.init_products_min, n_morph_);
+      const int hi = std::max(p_.init_products_max, lo);
+      Individual ind;
+      ind.model = create_random(rng, lo + rng.below(hi - lo + 1));
+      ind.dataset = island_datasets ? island_datasets[is] : 0;
+      prepare(&ind, is);
+      islands_[is].push_back(std::move(ind));
+    }
+  }
+  for (auto &isl : islands_)
+    for (Individual &ind : isl) pending_.push_back(&ind);
+}
+
+void Search::prepare(Individual *ind, int island) {
+  (void)island;
+  ind->id = next_id_++;
+  ind->error = -1;
+  ind->complexity = ind->model.complexity_in_use(n_morph_);
+  if (p_.dt_rule == 1) {
+    double dmax = 0;
+    for (const Gene &g : ind->model.genes) dmax = std::max(dmax, g.dif);
+    ind->dt = dmax > 0 ? std::min(p_.dt_scale / dmax, p_.dt_cap) : p_.dt_cap;
+  } else {
+    ind->dt = 0;
+  }
+}
+
+void Search::reproduce(int is) {
+  // SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:63-101)
+  Rng &rng = rngs_[is];
+  std::vector<int> &perm = perm_[is];
+  for (int i = (int)perm.size() - 1; i > 0; --i) std::swap(perm[i], perm[rng.below(i + 1)]);  // MathAlgo::shuffle
+  std::vector<Individual> &pop = islands_[is];
+  std::vector<Individual> &kids = children_[is];
+  kids.assign(pop.size(), Individual());
+  for (size_t i = 0; i + 1 < perm.size(); i += 2) {
+    const Individual &a = pop[perm[i]], &b = pop[perm[i + 1]];
+    Individual &k1 = kids[i], &k2 = kids[i + 1];
+    if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
+      cross(rng, a.model, b.model, &k1.model, &k2.model);
+    } else {
+      k1.model = a.model;
+      k2.model = b.model;
+    }
+    mutate(rng, &k1.model);
+    mutate(rng, &k2.model);
+    k1.parent_error = a.error;
+    k2.parent_error = b.error;
+    k1.dataset = a.dataset;
+    k2.dataset = b.dataset;
+    prepare(&k1, is);
+    prepare(&k2, is);
+  }
+}
+
+void Search::choose_next_generation(int is, std::vector<Individual *> *winners) {
+  // SearchAlgoDetCrowd::chooseNextGeneration (searchalgodetcrowd.cpp:103-147):
+  // each child competes with its own parent and wins ties
+  std::vector<Individual> &pop = islands_[is];
+  std::vector<Individual> &kids = children_[is];
+  const std::vector<int> &perm = perm_[is];
+  for (size_t i = 0; i < kids.size(); ++i) {
+    Individual &parent = pop[perm[i]];
+    if (kids[i].error <= parent.error) {
+      parent = std::move(kids[i]);
+      winners->push_back(&parent);
+    }
+  }
+  kids.clear();
+}
+
+void Search::migrate() {
+  Rng &rng = rngs_.back();
+  const int n = p_.n_islands;
+  if (n < 2 || p_.migration == GRN_MIGRATION_NONE) return;
+  if (p_.migration == GRN_MIGRATION_PAIRWISE) {
+    // createMigrationPartners + migrateIndividuals (search.cpp:263-311)
+    std::vector<int> ids(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    for (int i = n - 1; i > 0; --i) std::swap(ids[i], ids[rng.below(i + 1)]);
+    for (int i = 0; i + 1 < n; i += 2) {
+      std::vector<Individual> &g1 = islands_[ids[i]], &g2 = islands_[ids[i + 1]];
+      const int ds1 = g1.empty() ? 0 : g1[0].dataset, ds2 = g2.empty() ? 0 : g2[0].dataset;
+      std::vector<Individual> all;
+      for (Individual &x : g1) all.push_back(std::move(x));
+      for (Individual &x : g2) all.push_back(std::move(x));
+      std::vector<int> r(all.size());
+      std::iota(r.begin(), r.end(), 0);
+      for (int k = (int)r.size() - 1; k > 0; --k) std::swap(r[k], r[rng.below(k + 1)]);
+      const size_t half = g1.size();
+      std::vector<Individual> n1, n2;
+      for (size_t k = 0; k < r.size(); ++k) (k < half ? n1 : n2).push_back(std::move(all[r[k]]));
+      // an individual that changes island is scored on its new island's
+      // data from its next child on; its own error stays as measured
+      for (Individual &x : n1) x.dataset = ds1;
+      for (Individual &x : n2) x.dataset = ds2;
+      g1.swap(n1);
+      g2.swap(n2);
+    }
+    return;
+  }
+  // ring: the best fraction of island i replaces the worst of island i+1
+  const int k = std::max(1, (int)std::lround(p_.migration_fraction * p_.island_size));
+  std::vector<std::vector<Individual>> emigrants(n);
+  auto better = [](const Individual &a, const Individual &b) {
+    return a.error < b.error || (a.error == b.error && a.complexity < b.complexity);
+  };
+  for (int i = 0; i < n; ++i) {
+    std::vector<int> order(islands_[i].size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return better(islands_[i][a], islands_[i][b]); });
+    for (int j = 0; j < k && j < (int)order.size(); ++j) emigrants[i].push_back(islands_[i][order[j]]);
+  }
+  for (int i = 0; i < n; ++i) {
+    std::vector<Individual> &dst = islands_[(i + 1) % n];
+    std::vector<int> order(dst.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return better(dst[b], dst[a]); });
+    const int ds = dst.empty() ? 0 : dst[0].dataset;
+    for (int j = 0; j < (int)emigrants[i].size() && j < (int)order.size(); ++j) {
+      dst[order[j]] = emigrants[i][j];
+      dst[order[j]].dataset = ds;
+    }
+  }
+}
+
+void Search::update_pareto(const std::vector<Individual *> &fresh) {
+  // Search::recalculateParetoFront (search.cpp:349-377): sort by (error,
+  // complexity), keep strictly decreasing complexity, duplicates included
+  for (Individual *x : fresh) pareto_.push_back(*x);
+  std::stable_sort(pareto_.begin(), pareto_.end(), [](const Individual &a, const Individual &b) {
+    return a.error < b.error || (a.error == b.error && a.complexity < b.complexity);
+  });
+  std::vector<Individual> front;
+  for (Individual &x : pareto_) {
+    if (front.empty() || x.complexity < front.back().complexity ||
+        (x.complexity == front.back().complexity && x.error == front.back().error))
+      front.push_back(std::move(x));
+  }
+  pareto_.swap(front);
+}
+
+void Search::update_stats() {
+  // Generation::calcPopulSta (generation.cpp:42-72), over all islands
+  grn_generation_stats s{};
+  s.generation = generation_;
+  s.min_fit = std::numeric_limits<double>::infinity();
+  s.max_fit = -std::numeric_limits<double>::infinity();
+  s.best_complexity = 0;
+  double sum = 0;
+  int n = 0;
+  for (int is = 0; is < (int)islands_.size(); ++is)
+    for (const Individual &x : islands_[is]) {
+      sum += x.error;
+      ++n;
+      if (x.error < s.min_fit || (x.error == s.min_fit && x.complexity < s.best_complexity)) {
+        s.min_fit = x.error;
+        s.best_complexity = x.complexity;
+        s.best_island = is;
+      }
+      s.max_fit = std::max(s.max_fit, x.error);
+    }
+  s.mean_fit = n ? sum / n : 0;
+  s.pareto_size = (int)pareto_.size();
+  stats_ = s;
+}
+
+void Search::advance(const double *errors) {
+  for (size_t i = 0; i < pending_.size(); ++i) pending_[i]->error = errors[i];
+  const int n_evaluated = (int)pending_.size();
+  // the Pareto front sees the initial population and every selected child
+  // (Search::addNewIndividual, search.cpp:323-326)
+  std::vector<Individual *> winners;
+  if (generation_ == 0)
+    winners = pending_;
+  else
+    for (int is = 0; is < p_.n_islands; ++is) choose_next_generation(is, &winners);
+  update_pareto(winners);
+  ++generation_;
+  update_stats();
+  stats_.n_evaluated = n_evaluated;
+  if (p_.migration_period > 0 && generation_ % p_.migration_period == 0) migrate();
+  pending_.clear();
+  for (int is = 0; is < p_.n_islands; ++is) {
+    reproduce(is);
+    for (Individual &k : children_[is]) pending_.push_back(&k);
+  }
+}
+
+}  // namespace grn
+
+// ---- C ABI --------------------------------------------------------------------
+
+namespace {
+
+// A search plus the flat arrays grn_search_pending hands out.
+struct SearchHandle {
+  std::unique_ptr<grn::Search> s;
+  std::vector<grn_model> models;
+  std::vector<grn_gene> genes;
+  std::vector<grn_link> links;
+  std::vector<int> datasets;
+  std::vector<double> max_errors, dts;
+  std::vector<grn_result> results;
+  std::vector<double> errors;
+
+  void flatten() {
+    const auto &pend = s->pending();
+    size_t ng = 0, nl = 0;
+    for (const grn::Individual *x : pend) {
+      ng += x->model.genes.size();
+      nl += x->model.links.size();
+    }
+    genes.resize(ng);
+    links.resize(nl);
+    models.resize(pend.size());
+    datasets.resize(pend.size());
+    max_errors.resize(pend.size());
+    dts.resize(pend.size());
+    size_t gi = 0, li = 0;
+    for (size_t i = 0; i < pend.size(); ++i) {
+      const grn::Individual &x = *pend[i];
+      grn_model &m = models[i];
+      m.n_genes = (int)x.model.genes.size();
+      m.genes = genes.data() + gi;
+      m.n_links = (int)x.model.links.size();
+      m.links = links.data() + li;
+      for (const grn::Gene &g : x.model.genes) genes[gi++] = grn_gene{g.label, g.init, g.lim, g.deg, g.dif};
+      for (const grn::Link &l : x.model.links)
+        links[li++] = grn_link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and ? 1 : 0};
+      datasets[i] = x.dataset;
+      max_errors[i] = x.parent_error;
+      dts[i] = x.dt;
+    }
+  }
+};
+
+SearchHandle *as_search(void *p) { return static_cast<SearchHandle *>(p); }
+
+}  // namespace
+
+extern "C" {
+
+void grn_search_default_params(grn_search_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->n_islands = 1;
+  p->island_size = 64;
+  p->n_morphogens = 3;
+  p->n_max_products = -1;
+  p->n_max_links = -1;
+  p->init_products_min = 3;
+  p->init_products_max = 3;
+  p->migration = GRN_MIGRATION_PAIRWISE;
+  p->migration_period = 10;
+  p->migration_fraction = 0.1;
+  p->crossover_rate = 0.75;
+  p->lim_lo = 0, p->lim_hi = 1;
+  p->deg_lo = 0, p->deg_hi = 1;
+  p->dif_prob = 0.1, p->dif_lo = 0, p->dif_hi = 1;
+  p->k_lo = 0, p->k_hi = 1;
+  p->n_lo = 1, p->n_hi = 5;
+  p->dt_rule = 0;
+  p->dt_scale = 0.2;
+  p->dt_cap = 1.0;
+  p->seed = 1;
+}
+
+int grn_search_create(const grn_search_params *p, const int *island_datasets, void **out) {
+  if (!p || !out) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  *out = nullptr;
+  grn_search_params q = *p;
+  if (q.n_islands < 1 || q.island_size < 2 || q.n_morphogens < 1)
+    return grn::set_last_error(GRN_E_INVALID, "need n_islands >= 1, island_size >= 2, n_morphogens >= 1");
+  // -1 (no limit) becomes the capacity of the kernels' program tables
+  if (q.n_max_products < 0) q.n_max_products = GRN_PMAX_GENES;
+  if (q.n_max_links < 0) q.n_max_links = GRN_PMAX_LINKS;
+  if (q.n_max_products < q.n_morphogens || q.n_max_products > GRN_PMAX_GENES || q.n_max_links > GRN_PMAX_LINKS)
+    return grn::set_last_error(GRN_E_CAPACITY, "n_max_products must lie in [n_morphogens, %d], n_max_links in [0, %d]",
+                               GRN_PMAX_GENES, GRN_PMAX_LINKS);
+  if (q.init_products_max > q.n_max_products || q.init_products_min > q.init_products_max)
+    return grn::set_last_error(GRN_E_INVALID, "initial product range [%d, %d] outside [n_morphogens, n_max_products]",
+                               q.init_products_min, q.init_products_max);
+  if (q.migration < GRN_MIGRATION_PAIRWISE || q.migration > GRN_MIGRATION_NONE)
+    return grn::set_last_error(GRN_E_INVALID, "unknown migration %d", q.migration);
+  if (q.crossover_rate < 0 || q.crossover_rate > 1 || q.dif_prob < 0 || q.dif_prob > 1)
+    return grn::set_last_error(GRN_E_INVALID, "probabilities must lie in [0, 1]");
+  if (q.dt_rule == 1 && !(q.dt_scale > 0 && q.dt_cap > 0))
+    return grn::set_last_error(GRN_E_INVALID, "dt_rule 1 needs dt_scale > 0 and dt_cap > 0");
+  std::unique_ptr<SearchHandle> h(new SearchHandle);
+  h->s.reset(new grn::Search(q, island_datasets));
+  h->flatten();
+  *out = h.release();
+  return 0;
+}
+
+void grn_search_destroy(void *search) { delete as_search(search); }
+
+int grn_search_pending(void *search, int *n, const grn_model **models, const int **datasets,
+                       const double **max_errors, const double **dt) {
+  SearchHandle *h = as_search(search);
+  if (!h || !n) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  *n = (int)h->models.size();
+  if (models) *models = h->models.data();
+  if (datasets) *datasets = h->datasets.data();
+  if (max_errors) *max_errors = h->max_errors.data();
+  if (dt) *dt = h->dts.data();
+  return 0;
+}
+
+int grn_search_advance(void *search, const double *errors, grn_generation_stats *stats) {
+  SearchHandle *h = as_search(search);
+  if (!h || (!errors && !h->models.empty())) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  for (size_t i = 0; i < h->models.size(); ++i)
+    if (!(errors[i] >= 0) || std::isinf(errors[i]))
+      return grn::set_last_error(GRN_E_INVALID, "error %zu is %g; errors must be finite and >= 0", i, errors[i]);
+  h->s->advance(errors);
+  h->flatten();
+  if (stats) *stats = h->s->stats();
+  return 0;
+}
+
+int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats) {
+  SearchHandle *h = as_search(search);
+  if (!h || !ctx) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  const int n = (int)h->models.size();
+  h->results.resize(n);
+  h->errors.resize(n);
+  int rc = grn_evaluate_batch(ctx, n, h->datasets.data(), h->models.data(), h->max_errors.data(), h->dts.data(),
+                              path, h->results.data());
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) h->errors[i] = h->results[i].fitness;
+  return grn_search_advance(search, h->errors.data(), stats);
+}
+
+int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links, int link_cap,
+                    int *n_links, double *error) {
+  SearchHandle *h = as_search(search);
+  if (!h || !n_genes || !n_links) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  const grn::Individual *best = nullptr;
+  for (const auto &isl : h->s->islands())
+    for (const grn::Individual &x : isl) {
+      if (x.error < 0) continue;
+      if (!best || x.error < best->error || (x.error == best->error && x.complexity < best->complexity)) best = &x;
+    }
+  if (!best) return grn::set_last_error(GRN_E_INVALID, "no individual has been evaluated yet");
+  *n_genes = (int)best->model.genes.size();
+  *n_links = (int)best->model.links.size();
+  if (error) *error = best->error;
+  if (*n_genes > gene_cap || *n_links > link_cap || (*n_genes && !genes) || (*n_links && !links))
+    return grn::set_last_error(GRN_E_CAPACITY, "best model has %d genes and %d links", *n_genes, *n_links);
+  for (int i = 0; i < *n_genes; ++i) {
+    const grn::Gene &g = best->model.genes[i];
+    genes[i] = grn_gene{g.label, g.init, g.lim, g.deg, g.dif};
+  }
+  for (int i = 0; i < *n_links; ++i) {
+    const grn::Link &l = best->model.links[i];
+    links[i] = grn_link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and ? 1 : 0};
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// grn_search.hpp -- the evolutionary search that calls the simulation path:
+// island populations evolved by deterministic crowding with the reference's
+// crossover and mutation operators, plus migration between islands.
+//
+// Reference behaviour restated (paths relative to the reference tree):
+//   random model ............ Src/Model/model.cpp:285-315, modelprod.cpp:14-24,
+//                             modellink.cpp:10-22
+//   crossover ............... model.cpp:21-185
+//   mutation ................ model.cpp:317-333,472-547, modelprod.cpp:43-59,
+//                             modellink.cpp:24-40
+//   excess removal .......... model.cpp:549-574
+//   deterministic crowding .. Src/Search/searchalgodetcrowd.cpp:54-147
+//   pairwise migration ...... Src/Search/search.cpp:263-311
+//   Pareto front ............ search.cpp:349-377, individual.cpp:139-143
+//   generation statistics ... Src/Search/generation.cpp:42-72
+// The reference draws from a per-translation-unit std::mt19937_64 seeded by
+// std::random_device (mathalgo.h:222-230), so its runs cannot be replayed;
+// here every island draws from its own seeded xoshiro256** stream.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "grn.h"
+#include "grn_model.hpp"
+
+namespace grn {
+
+// Sets the calling thread's grn_last_error() text and returns code (defined
+// with the rest of the C ABI in grn_host.cpp).
+int set_last_error(int code, const char *fmt, ...);
+
+// xoshiro256** (Blackman & Vigna) seeded through splitmix64.
+class Rng {
+ public:
+  explicit Rng(uint64_t seed = 1);
+  uint64_t next();
+  double uniform01();                   // [0, 1)
+  double uniform(double lo, double hi);  // [lo, hi)
+  int below(int n);                     // [0, n), n > 0
+  bool coin() { return (next() >> 63) != 0; }
+
+ private:
+  uint64_t s_[4];
+};
+
+struct Individual {
+  Model model;
+  double error = -1;           // -1 until evaluated
+  double parent_error = 1000;  // maxError of its evaluation (individual.cpp:19-25)
+  int complexity = 0;          // Model::calcComplexityInUse
+  int dataset = 0;
+  double dt = 0;               // per-model time step (0: the dataset's)
+  uint64_t id = 0;
+};
+
+class Search {
+ public:
+  Search(const grn_search_params &p, const int *island_datasets);
+
+  // Models that need an error now (children or the initial population).
+  int n_pending() const { return (int)pending_.size(); }
+  const std::vector<Individual *> &pending() const { return pending_; }
+  // Receives the errors of pending(), in order, then selects, migrates,
+  // reproduces and refills pending().
+  void advance(const double *errors);
+
+  const grn_search_params &params() const { return p_; }
+  int generation() const { return generation_; }
+  const std::vector<std::vector<Individual>> &islands() const { return islands_; }
+  const std::vector<Individual> &pareto() const { return pareto_; }
+  grn_generation_stats stats() const { return stats_; }
+
+  // Operators, public for tests.
+  Model create_random(Rng &rng, int n_products);
+  void cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model *c2);
+  void mutate(Rng &rng, Model *m);
+  void remove_excess(Rng &rng, Model *m);
+
+ private:
+  Gene random_gene(Rng &rng, int label);
+  Link random_link(Rng &rng, int regulator, int regulated);
+  void mutate_gene_params(Rng &rng, Gene *g);
+  void mutate_link_params(Rng &rng, Link *l);
+  int new_label() { return ++label_counter_; }
+  void prepare(Individual *ind, int island);
+  void reproduce(int island);
+  void choose_next_generation(int island, std::vector<Individual *> *winners);
+  void migrate();
+  void update_pareto(const std::vector<Individual *> &fresh);
+  void update_stats();
+
+  grn_search_params p_;
+  int n_morph_ = 0;
+  int generation_ = 0;  // 0 = initial population pending
+  int label_counter_ = 10;  // Model::createNewLabel starts after 10 (model.cpp:373-379)
+  uint64_t next_id_ = 1;
+  std::vector<Rng> rngs_;  // one stream per island, plus one for migration
+  std::vector<std::vector<Individual>> islands_;
+  std::vector<std::vector<Individual>> children_;
+  std::vector<std::vector<int>> perm_;  // shuffled parent order per island
+  std::vector<Individual *> pending_;
+  std::vector<Individual> pareto_;
+  grn_generation_stats stats_{};
+};
+
+}  // namespace grn

@@ -40,7 +40,11 @@ EXPORTED = [
     "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
     "grn_get_stats", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
+    "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
+    "grn_search_advance", "grn_search_run_generation", "grn_search_best",
 ]
+
+MIGRATION_PAIRWISE, MIGRATION_RING, MIGRATION_NONE = 0, 1, 2
 
 
 class GrnError(RuntimeError):
@@ -76,6 +80,30 @@ class _Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
                 ("cell_steps", ctypes.c_double), ("alg_flops", ctypes.c_double),
                 ("n_launches", ctypes.c_int), ("n_models", ctypes.c_int)]
+
+
+class _SearchParams(ctypes.Structure):
+    _fields_ = [("n_islands", ctypes.c_int), ("island_size", ctypes.c_int),
+                ("n_morphogens", ctypes.c_int), ("n_max_products", ctypes.c_int),
+                ("n_max_links", ctypes.c_int), ("init_products_min", ctypes.c_int),
+                ("init_products_max", ctypes.c_int), ("migration", ctypes.c_int),
+                ("migration_period", ctypes.c_int), ("migration_fraction", ctypes.c_double),
+                ("crossover_rate", ctypes.c_double),
+                ("lim_lo", ctypes.c_double), ("lim_hi", ctypes.c_double),
+                ("deg_lo", ctypes.c_double), ("deg_hi", ctypes.c_double),
+                ("dif_prob", ctypes.c_double), ("dif_lo", ctypes.c_double),
+                ("dif_hi", ctypes.c_double), ("k_lo", ctypes.c_double), ("k_hi", ctypes.c_double),
+                ("n_lo", ctypes.c_double), ("n_hi", ctypes.c_double),
+                ("dt_rule", ctypes.c_int), ("dt_scale", ctypes.c_double), ("dt_cap", ctypes.c_double),
+                ("seed", ctypes.c_ulonglong)]
+
+
+class _GenerationStats(ctypes.Structure):
+    _fields_ = [("generation", ctypes.c_int), ("n_evaluated", ctypes.c_int),
+                ("min_fit", ctypes.c_double), ("mean_fit", ctypes.c_double),
+                ("max_fit", ctypes.c_double), ("best_complexity", ctypes.c_int),
+                ("best_island", ctypes.c_int), ("pareto_size", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
 
 RESULT_DTYPE = np.dtype({"names": ["fitness", "raw_error", "change", "n_products", "steps",
@@ -116,8 +144,19 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                      ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     lib.grn_model_info.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, ip, ip, dp]
     lib.grn_copy_probe.argtypes = [ctypes.c_long, dp]
+    lib.grn_search_default_params.argtypes = [ctypes.POINTER(_SearchParams)]
+    lib.grn_search_default_params.restype = None
+    lib.grn_search_create.argtypes = [ctypes.POINTER(_SearchParams), vp, ctypes.POINTER(vp)]
+    lib.grn_search_destroy.argtypes = [vp]
+    lib.grn_search_destroy.restype = None
+    lib.grn_search_pending.argtypes = [vp, ip, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                       ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    lib.grn_search_advance.argtypes = [vp, vp, ctypes.POINTER(_GenerationStats)]
+    lib.grn_search_run_generation.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_GenerationStats)]
+    lib.grn_search_best.argtypes = [vp, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip, dp]
     for name in EXPORTED:
-        if name not in ("grn_last_error", "grn_destroy", "grn_ceil_s", "grn_abi_version"):
+        if name not in ("grn_last_error", "grn_destroy", "grn_ceil_s", "grn_abi_version",
+                        "grn_search_default_params", "grn_search_destroy"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -286,6 +325,100 @@ class Context:
         s = _Stats()
         _check(self._lib.grn_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+
+# ---- the search that calls the path ---------------------------------------------
+
+def search_params(**overrides) -> dict:
+    """The reference's search defaults (grn_search_default_params) with overrides."""
+    p = _SearchParams()
+    load_library().grn_search_default_params(ctypes.byref(p))
+    out = {k: getattr(p, k) for k, _ in _SearchParams._fields_}
+    for k, v in overrides.items():
+        if k not in out:
+            raise KeyError(k)
+        out[k] = v
+    return out
+
+
+def _models_from_c(n: int, ptr: int) -> list:
+    arr = (_Model * n).from_address(ptr) if n else []
+    out = []
+    for cm in arr:
+        g = np.zeros(cm.n_genes, GENE_DTYPE)
+        l = np.zeros(cm.n_links, LINK_DTYPE)
+        if cm.n_genes:
+            ctypes.memmove(g.ctypes.data, cm.genes, cm.n_genes * GENE_DTYPE.itemsize)
+        if cm.n_links:
+            ctypes.memmove(l.ctypes.data, cm.links, cm.n_links * LINK_DTYPE.itemsize)
+        out.append(Model(g, l))
+    return out
+
+
+class Search:
+    """Island search by deterministic crowding (Src/Search/search.cpp,
+    searchalgodetcrowd.cpp): the caller of the simulation path."""
+
+    def __init__(self, island_datasets=None, **params):
+        lib = load_library()
+        self._lib = lib
+        self.params = search_params(**params)
+        cp = _SearchParams(**self.params)
+        ids = None
+        if island_datasets is not None:
+            ids = np.ascontiguousarray(np.broadcast_to(np.asarray(island_datasets, np.int32),
+                                                       (self.params["n_islands"],)))
+        h = ctypes.c_void_p()
+        _check(lib.grn_search_create(ctypes.byref(cp), None if ids is None else ids.ctypes.data,
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.grn_search_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pending(self):
+        """(models, datasets, max_errors, dt) of the individuals awaiting an error."""
+        n = ctypes.c_int()
+        pm, pd, pe, pt = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(self._lib.grn_search_pending(self._h, ctypes.byref(n), ctypes.byref(pm), ctypes.byref(pd),
+                                            ctypes.byref(pe), ctypes.byref(pt)))
+        k = n.value
+        models = _models_from_c(k, pm.value)
+
+        def arr(ptr, ct, dtype):
+            if not k:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array((ct * k).from_address(ptr.value)).astype(dtype)
+        return (models, arr(pd, ctypes.c_int, np.int32), arr(pe, ctypes.c_double, np.float64),
+                arr(pt, ctypes.c_double, np.float64))
+
+    def advance(self, errors) -> dict:
+        e = np.ascontiguousarray(errors, dtype=np.float64)
+        st = _GenerationStats()
+        _check(self._lib.grn_search_advance(self._h, e.ctypes.data if e.size else None, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}
+
+    def run_generation(self, ctx: "Context", path: int = PATH_AUTO) -> dict:
+        st = _GenerationStats()
+        _check(self._lib.grn_search_run_generation(self._h, ctx._h, path, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}
+
+    def best(self):
+        """(Model, error) of the lowest-error individual (ties: lower complexity)."""
+        g = np.zeros(MAX_GENES * 16, GENE_DTYPE)
+        l = np.zeros(MAX_LINKS * 16, LINK_DTYPE)
+        ng, nl, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.grn_search_best(self._h, g.ctypes.data, len(g), ctypes.byref(ng), l.ctypes.data,
+                                         len(l), ctypes.byref(nl), ctypes.byref(e)))
+        return Model(g[:ng.value].copy(), l[:nl.value].copy()), e.value
 
 
 def copy_probe(nbytes: int) -> float:

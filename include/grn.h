@@ -196,6 +196,81 @@ int grn_model_format(const grn_model *model, int precision, char *buf, long cap,
 int grn_model_info(const grn_model *model, int n_morphogens, int *genes_in_use, int *hill_links,
                    double *flops_per_cell_step);
 
+/* ---- evolutionary search: the caller of the path ------------------------
+ * Island populations evolved by deterministic crowding with the reference's
+ * operators (Src/Search/searchalgodetcrowd.cpp:54-147, Src/Model/model.cpp
+ * 21-185, 285-333, 472-574) and migration between islands
+ * (Src/Search/search.cpp:263-311).  The search is driven in phases so any
+ * evaluator can serve it: grn_search_pending() lists the models that need an
+ * error, grn_search_advance() takes their errors and produces the next
+ * generation's children; grn_search_run_generation() does both with
+ * grn_evaluate_batch on a context. */
+
+typedef enum {
+  GRN_MIGRATION_PAIRWISE = 0, /* random island pairs, union shuffled and split (search.cpp:263-311) */
+  GRN_MIGRATION_RING = 1,     /* best fraction of island i replaces the worst of island i+1 */
+  GRN_MIGRATION_NONE = 2
+} grn_migration;
+
+typedef struct {
+  int n_islands;            /* SearchParams.nDemes */
+  int island_size;          /* SearchParams.demesSize, rounded up to even */
+  int n_morphogens;         /* nInputMorphogens + nTargetMorphogens (= nMinProducts) */
+  int n_max_products;       /* SimParams.nMaxProducts, -1 = no limit */
+  int n_max_links;          /* SimParams.nMaxLinks, -1 = no limit */
+  int init_products_min;    /* Model::createRandom sizes of the initial population */
+  int init_products_max;
+  int migration;            /* grn_migration */
+  int migration_period;     /* generations between migrations (SearchParams.migrationPeriod) */
+  double migration_fraction;/* ring: fraction of an island that emigrates */
+  double crossover_rate;    /* 0.75 in the reference (searchalgodetcrowd.cpp:74) */
+  /* distributions of new genes and links and of parameter mutations; the
+   * reference draws lim, deg, dif, K from U(0,1), |n| from U(1,5), and makes
+   * a new gene diffusing with probability 0.1 (modelprod.cpp:14-24) */
+  double lim_lo, lim_hi, deg_lo, deg_hi;
+  double dif_prob, dif_lo, dif_hi;
+  double k_lo, k_hi, n_lo, n_hi;
+  int dt_rule;              /* 0: the dataset's dt; 1: dt_scale / max dif, at most dt_cap */
+  double dt_scale, dt_cap;
+  unsigned long long seed;
+} grn_search_params;
+
+typedef struct {
+  int generation;           /* generations completed */
+  int n_evaluated;          /* models evaluated by the last advance */
+  double min_fit, mean_fit, max_fit; /* Generation::calcPopulSta over all islands */
+  int best_complexity;      /* complexity in use of the best individual */
+  int best_island;
+  int pareto_size;          /* size of the (error, complexity) Pareto front */
+  int reserved;
+} grn_generation_stats;
+
+/* The reference's defaults for everything grn_search_params holds. */
+void grn_search_default_params(grn_search_params *p);
+
+/* island_datasets[i]: the dataset id every model of island i is evaluated on. */
+int grn_search_create(const grn_search_params *p, const int *island_datasets, void **search);
+void grn_search_destroy(void *search);
+
+/* Models that need an error now, with the dataset, maxError (the parent's
+ * error, 1000 for the initial population, individual.cpp:19-25) and time
+ * step (0 = the dataset's) of each.  Pointers stay valid until the next
+ * grn_search_advance. */
+int grn_search_pending(void *search, int *n, const grn_model **models, const int **datasets,
+                       const double **max_errors, const double **dt);
+
+/* Takes the pending models' errors (fitness as grn_result.fitness returns
+ * it), selects (a child replaces its parent when child <= parent,
+ * searchalgodetcrowd.cpp:119,128), migrates when due and reproduces. */
+int grn_search_advance(void *search, const double *errors, grn_generation_stats *stats);
+
+/* grn_search_pending + grn_evaluate_batch on ctx + grn_search_advance. */
+int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats);
+
+/* The lowest-error individual over all islands (ties: lower complexity). */
+int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                    int link_cap, int *n_links, double *error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,247 @@
+"""The evolutionary search that calls the path (grn_search_* of include/grn.h).
+
+The reference draws its random numbers from an unseeded std::mt19937_64
+(Src/Common/mathalgo.h:222-230), so its runs cannot be replayed and there are
+no golden search trajectories.  These tests pin what the reference's code
+fixes regardless of the random stream:
+
+* the initial population of SearchAlgoDetCrowd::calcInitialPopulation
+  (searchalgodetcrowd.cpp:54-61, model.cpp:285-315, modelprod.cpp:14-24,61-70);
+* the coherence the reference asserts in debug builds after cross, mutate and
+  createRandom (model.cpp:67-82,302-311,536-546): every link joins products
+  of its model; plus one link per (regulator, regulated) pair
+  (addOrReplaceRandomLink, model.cpp:449-453), morphogen products never
+  removed (model.cpp:509,557-558) and the in-use caps of removeExcess
+  (model.cpp:550-574);
+* deterministic crowding: a child's maxError is its own parent's error
+  (individual.cpp:19-25) and replaces that parent only when not worse
+  (searchalgodetcrowd.cpp:119,128), so no island slot ever gets worse and
+  the population's best and mean never rise;
+* the 1 % duplicate-product rate of Model::mutate (model.cpp:505-506);
+* pairwise full-mix migration (search.cpp:263-311) and the ring variant;
+* a search driven end to end by the FP64 oracle as its evaluator.
+Everything here runs on the CPU: the errors come from a deterministic fake
+fitness or from the oracle.
+"""
+import numpy as np
+import pytest
+
+import grn
+import grn_oracle as ora
+
+
+def fake_error(m, dataset=0):
+    """A deterministic error that depends on every parameter of the model."""
+    g, l = m.genes, m.links
+    v = abs(float(np.sum(g["lim"] * (1 + g["label"] % 7))) - 1.7) + 0.01 * len(l)
+    v += 0.1 * abs(float(np.sum(l["dis_const"] * l["hill_coef"])) - 0.3)
+    return 1.0 + (v % 1.0) * 0.999 + 1000.0 * dataset
+
+
+def drive(search, gens, fitness=fake_error):
+    """Runs gens generations with fitness as evaluator; returns (stats, pending) per generation."""
+    hist = []
+    for _ in range(gens):
+        models, ds, me, dt = search.pending()
+        errs = np.array([fitness(m, d) for m, d in zip(models, ds)])
+        st = search.advance(errs)
+        hist.append((st, search.pending(), errs))
+    return hist
+
+
+def check_coherent(m, n_morph, max_products, max_links):
+    labels = [int(x) for x in m.genes["label"]]
+    assert len(set(labels)) == len(labels), "duplicate product labels"
+    assert set(range(n_morph)) <= set(labels), "a morphogen product was removed"
+    pairs = set()
+    for r, t in zip(m.links["regulator"], m.links["regulated"]):
+        assert int(r) in labels and int(t) in labels, "link to a missing product"
+        assert (int(r), int(t)) not in pairs, "two links with one (regulator, regulated) pair"
+        pairs.add((int(r), int(t)))
+    in_use = set(ora.labels_in_use(m.genes, m.links, n_morph))
+    assert len(in_use) <= max_products
+    links_in_use = sum(1 for r, t in pairs if r in in_use and t in in_use)
+    assert links_in_use <= max_links
+
+
+def test_initial_population():
+    s = grn.Search(n_islands=3, island_size=33, n_morphogens=3, seed=7)
+    models, ds, me, dt = s.pending()
+    assert len(models) == 3 * 34  # island size rounded up to even (searchalgodetcrowd.cpp:33-35)
+    assert np.all(me == 1000.0)   # Individual without parents (individual.cpp:23-24)
+    assert np.all(ds == 0) and np.all(dt == 0)
+    n_links = []
+    for m in models:
+        assert list(m.genes["label"]) == [0, 1, 2]
+        assert list(m.genes["init"]) == [-1.0, -2.0, -3.0]  # zeroInit tags (modelprod.cpp:61-70)
+        for k in ("lim", "deg", "dif"):
+            assert np.all((m.genes[k] >= 0) & (m.genes[k] < 1))
+        assert np.all((m.links["dis_const"] >= 0) & (m.links["dis_const"] < 1))
+        n = np.abs(m.links["hill_coef"])
+        assert np.all((n >= 1) & (n < 5))
+        # repressors are AND links (modellink.cpp:21)
+        assert np.all(m.links["is_and"][m.links["hill_coef"] < 0] == 1)
+        check_coherent(m, 3, 16, 96)
+        n_links.append(len(m.links))
+    # each product regulates randInt(1 + n) = 0..3 targets, duplicates replaced
+    assert 0 <= min(n_links) and max(n_links) <= 9
+    assert 2.0 < np.mean(n_links) < 4.0
+    # about 10 % of products diffuse (modelprod.cpp:19-22)
+    dif = np.concatenate([m.genes["dif"] for m in models])
+    assert 0.04 < np.mean(dif > 0) < 0.18
+
+
+def test_replay_is_deterministic():
+    a = grn.Search(n_islands=2, island_size=40, seed=11)
+    b = grn.Search(n_islands=2, island_size=40, seed=11)
+    c = grn.Search(n_islands=2, island_size=40, seed=12)
+    ha, hb, hc = drive(a, 6), drive(b, 6), drive(c, 6)
+    for (sa, pa, _), (sb, pb, _) in zip(ha, hb):
+        assert sa == sb
+        assert [m.to_text(17) for m in pa[0]] == [m.to_text(17) for m in pb[0]]
+        assert np.array_equal(pa[2], pb[2])
+    assert [m.to_text(17) for m in ha[-1][1][0]] != [m.to_text(17) for m in hc[-1][1][0]]
+
+
+@pytest.mark.parametrize("migration", [grn.MIGRATION_PAIRWISE, grn.MIGRATION_RING, grn.MIGRATION_NONE])
+def test_invariants_over_generations(migration):
+    s = grn.Search(n_islands=4, island_size=48, n_morphogens=3, n_max_products=6, n_max_links=8,
+                   migration=migration, migration_period=3, seed=3)
+    submitted = set()
+    prev = None
+    for st, (models, ds, me, dt), errs in drive(s, 30):
+        submitted.update(errs.tolist())
+        for m in models:
+            check_coherent(m, 3, 6, 8)
+        # a child's maxError is an error its parent was given
+        assert set(me.tolist()) <= submitted
+        assert st["n_evaluated"] == len(errs)
+        assert st["min_fit"] <= st["mean_fit"] <= st["max_fit"]
+        if prev is not None:
+            # no slot gets worse; migration only moves individuals
+            assert st["min_fit"] <= prev["min_fit"]
+            assert st["mean_fit"] <= prev["mean_fit"] + 1e-12
+        assert st["pareto_size"] >= 1
+        prev = st
+    assert prev["generation"] == 30
+    best, err = s.best()
+    assert err == prev["min_fit"]
+    assert fake_error(best) == err
+
+
+def test_duplicate_product_rate():
+    # with no crossover every child is its parent mutated once; a child of a
+    # 3-product parent holds a new label (> 10, Model::createNewLabel) with
+    # probability 1 - 0.99^3 ~= 2.97 %
+    s = grn.Search(n_islands=1, island_size=8192, n_morphogens=3, crossover_rate=0.0, seed=5)
+    drive(s, 1)
+    models = s.pending()[0]
+    frac = np.mean([np.any(m.genes["label"] > 10) for m in models])
+    assert 0.022 < frac < 0.038
+    assert all(np.all((m.genes["label"] < 3) | (m.genes["label"] > 10)) for m in models)
+
+
+def _cross_island_children(pending, island_of_error):
+    models, ds, me, dt = pending
+    src = np.array([island_of_error(e) for e in me])
+    return ds, src
+
+
+def test_ring_migration_moves_the_best_fraction():
+    n_isl, size, frac = 4, 40, 0.1
+    s = grn.Search(island_datasets=list(range(n_isl)), n_islands=n_isl, island_size=size,
+                   migration=grn.MIGRATION_RING, migration_period=2, migration_fraction=frac, seed=9)
+    hist = drive(s, 2)
+    # before the first migration every child's parent lived on its island
+    ds, src = _cross_island_children(hist[0][1], lambda e: int(e // 1000))
+    assert np.all(ds == src)
+    # after it, island i+1 holds exactly round(frac * size) children of island i's best
+    ds, src = _cross_island_children(hist[1][1], lambda e: int(e // 1000))
+    k = round(frac * size)
+    for i in range(n_isl):
+        dst = (i + 1) % n_isl
+        assert np.sum((ds == dst) & (src == i)) == k
+        assert np.sum(ds == dst) == size
+    assert np.sum(ds != src) == n_isl * k
+
+
+def test_pairwise_migration_mixes_partner_islands():
+    n_isl, size = 4, 64
+    s = grn.Search(island_datasets=list(range(n_isl)), n_islands=n_isl, island_size=size,
+                   migration=grn.MIGRATION_PAIRWISE, migration_period=2, seed=21)
+    hist = drive(s, 2)
+    ds, src = _cross_island_children(hist[0][1], lambda e: int(e // 1000))
+    assert np.all(ds == src)
+    ds, src = _cross_island_children(hist[1][1], lambda e: int(e // 1000))
+    for i in range(n_isl):
+        assert np.sum(ds == i) == size
+        partners = set(src[ds == i].tolist())
+        assert len(partners) == 2 and i in partners  # own island plus one partner
+        other = (partners - {i}).pop()
+        # the partner's children came from the same pair
+        assert set(src[ds == other].tolist()) == {i, other}
+        # full mix: each island keeps about half of its own
+        assert 0.25 * size < np.sum((ds == i) & (src == i)) < 0.75 * size
+
+
+def test_no_migration_keeps_islands_apart():
+    s = grn.Search(island_datasets=[0, 1, 2], n_islands=3, island_size=32,
+                   migration=grn.MIGRATION_NONE, migration_period=1, seed=2)
+    for _, pend, _ in drive(s, 5):
+        ds, src = _cross_island_children(pend, lambda e: int(e // 1000))
+        assert np.all(ds == src)
+
+
+def test_dt_rule():
+    s = grn.Search(n_islands=1, island_size=256, dt_rule=1, dt_scale=0.2, dt_cap=1.0, dif_prob=0.5,
+                   seed=4)
+    models, _, _, dt = s.pending()
+    for m, d in zip(models, dt):
+        dmax = float(np.max(m.genes["dif"]))
+        want = min(0.2 / dmax, 1.0) if dmax > 0 else 1.0
+        assert d == pytest.approx(want, rel=1e-15)
+
+
+@pytest.mark.parametrize("bad", [dict(n_islands=0), dict(island_size=1), dict(n_max_products=17),
+                                 dict(n_max_links=97), dict(n_max_products=2, n_morphogens=3),
+                                 dict(migration=5), dict(crossover_rate=1.5),
+                                 dict(init_products_min=5, init_products_max=4),
+                                 dict(dt_rule=1, dt_cap=0.0)])
+def test_invalid_params_are_rejected(bad):
+    with pytest.raises(grn.GrnError):
+        grn.Search(**bad)
+
+
+def test_advance_rejects_bad_errors():
+    s = grn.Search(n_islands=1, island_size=4)
+    with pytest.raises(grn.GrnError):
+        s.advance(np.array([0.1, -1.0, 0.2, 0.3]))
+    with pytest.raises(grn.GrnError):
+        s.advance(np.array([0.1, np.nan, 0.2, 0.3]))
+    with pytest.raises(grn.GrnError):
+        s.best()  # nothing evaluated yet
+
+
+def test_search_driven_by_the_oracle():
+    # a tiny morphogen problem: 2 inputs (x and y gradients), 1 target
+    W = H = 8
+    x = np.linspace(0, 1, W)
+    init = np.zeros((1, 2, H, W))
+    init[0, 0] = x[None, :]
+    init[0, 1] = x[:, None]
+    target = np.zeros((1, 1, H, W))
+    target[0, 0] = 0.5 * x[None, :] * x[:, None]
+    p = ora.params(W, H, 2, 1, 60, 0.05, fitness_kind=1)
+    s = grn.Search(n_islands=2, island_size=32, n_morphogens=3, n_max_products=6, n_max_links=10,
+                   migration_period=2, seed=1)
+    prev = None
+    for _ in range(4):
+        models, ds, me, dt = s.pending()
+        res = ora.evaluate_batch(p, init, target, models, max_errors=me, n_threads=4)
+        st = s.advance(res["fitness"])
+        if prev is not None:
+            assert st["min_fit"] <= prev["min_fit"]
+        prev = st
+    best, err = s.best()
+    r, _ = ora.evaluate(p, init, target, best.genes, best.links)
+    assert r["fitness"] == err
