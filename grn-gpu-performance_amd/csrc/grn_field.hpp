@@ -401,13 +401,16 @@ __global__ __launch_bounds__(NTMAX) void field_kernel(GrnLaunch a, const GrnProg
   __shared__ float s_wmax[NTMAX / 64];
   __shared__ double s_wsum[NTMAX / 64];
   Shared sh{lds, s_wmax, s_wsum};
+  GrnQueue *q = a.queue;
+  const int n = __builtin_amdgcn_readfirstlane(q->n);
+  const int *ord = order + __builtin_amdgcn_readfirstlane(q->begin);
   for (;;) {
-    if (threadIdx.x == 0) s_model = atomicAdd(a.counter, 1);
+    if (threadIdx.x == 0) s_model = atomicAdd(&q->head, 1);
     __syncthreads();
     const int m = __builtin_amdgcn_readfirstlane(s_model);
     __syncthreads();
-    if (m >= a.n) break;
-    const int pi = __builtin_amdgcn_readfirstlane(order[m]);
+    if (m >= n) break;
+    const int pi = __builtin_amdgcn_readfirstlane(ord[m]);
     const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
     dispatch_g<WC, CPT, NTMAX, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
     __syncthreads();

@@ -98,14 +98,23 @@ struct Ctx {
   int n_cu = 0;
   size_t max_lds = 0;
   std::vector<HostDataset> datasets;
-  DevBuf d_datasets, d_progs, d_order, d_maxerr, d_out, d_counter, d_fields;
+  DevBuf d_datasets, d_progs, d_order, d_maxerr, d_out, d_fields;
   std::vector<const GrnKernelEntry *> kernels;
   std::map<const void *, size_t> lds_set;  // dynamic-LDS attribute already applied
   DevBuf d_scratch;                         // halo-tiled state slots
+  DevBuf d_batch;                           // GrnBatch: per-launch work queues
   int tile_halo = 4;                        // Euler steps per tile load
+  int launch_mode = GRN_LAUNCH_EAGER;
+  std::map<std::vector<long>, hipGraphExec_t> graphs;  // by launch layout
   grn_stats stats{};
 
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+
   ~Ctx() {
+    drop_graphs();
     for (HostDataset &d : datasets) {
       if (d.d_init) (void)hipFree(d.d_init);
       if (d.d_target) (void)hipFree(d.d_target);
@@ -293,18 +302,11 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   if ((rc = c->d_order.reserve(sizeof(int) * std::max(n, 1)))) return rc;
   if ((rc = c->d_maxerr.reserve(sizeof(double) * std::max(n, 1)))) return rc;
   if ((rc = c->d_out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
-  if ((rc = c->d_counter.reserve(sizeof(int) * 64))) return rc;
+  if ((rc = c->d_batch.reserve(sizeof(GrnBatch)))) return rc;
   if (fields_host) {
     if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
     for (int i = 1; i < n; ++i)
       if (!same_class(i, 0)) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
-  }
-  hipStream_t st = c->stream;
-  if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(c->d_progs.p, progs.data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(c->d_order.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(c->d_maxerr.p, maxerr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(c->d_out.p, 0, sizeof(GrnModelOut) * n, st));
   }
 
   // One persistent launch per kernel class.
@@ -315,12 +317,19 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     groups.push_back({i, j});
     i = j;
   }
-  if ((int)groups.size() > 64) return fail(GRN_E_CAPACITY, "too many kernel classes in one batch");
-  HIP_TRY(hipMemsetAsync(c->d_counter.p, 0, sizeof(int) * 64, st));
-  HIP_TRY(hipEventRecord(c->ev0, st));
-  int launches = 0;
+  if ((int)groups.size() > GRN_MAX_QUEUES) return fail(GRN_E_CAPACITY, "too many kernel classes in one batch");
+  GrnBatch batch{};
+  batch.n_total = n;
+  batch.n_queues = (int)groups.size();
+  struct Launch {
+    KernelChoice kc;
+    int grid;
+    GrnLaunch L;
+  };
+  std::vector<Launch> plan;
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     const int b = groups[gi].first, e = groups[gi].second;
+    batch.q[gi] = GrnQueue{0, e - b, b, 0};
     // the class's block size and LDS: the largest over its members
     KernelChoice kc = items[idx[b]].kc;
     for (int j = b; j < e; ++j) {
@@ -337,11 +346,13 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
     if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
     const int grid = std::min(e - b, per_cu * c->n_cu);
-    if (kc.slot_floats > 0) {
-      int rc2 = c->d_scratch.reserve(sizeof(float) * (size_t)kc.slot_floats * grid);
-      if (rc2) return rc2;
-    }
-    GrnLaunch L{};
+    if (kc.slot_floats > 0 && (rc = c->d_scratch.reserve(sizeof(float) * (size_t)kc.slot_floats * grid))) return rc;
+    plan.push_back({kc, grid, GrnLaunch{}});
+  }
+  // Every buffer is sized now: the launch arguments below stay valid.
+  for (size_t gi = 0; gi < plan.size(); ++gi) {
+    const KernelChoice &kc = plan[gi].kc;
+    GrnLaunch &L = plan[gi].L;
     L.scratch = (float *)c->d_scratch.p;
     L.slot_floats = kc.slot_floats;
     L.tile_w = kc.tile.tw;
@@ -350,23 +361,82 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     L.tile_sy = kc.tile.sy;
     L.tile_halo = kc.tile.halo;
     L.progs = (const GrnProgram *)c->d_progs.p;
-    L.order = (const int *)c->d_order.p + b;
-    L.n = e - b;
-    L.counter = (int *)c->d_counter.p + gi;
+    L.queue = &((GrnBatch *)c->d_batch.p)->q[gi];
     L.datasets = (const GrnDataset *)c->d_datasets.p;
     L.max_err = (const double *)c->d_maxerr.p;
     L.out = (GrnModelOut *)c->d_out.p;
     L.fields = fields_host ? (double *)c->d_fields.p : nullptr;
     L.field_stride = field_stride;
     L.field_exp = field_exp;
-    const GrnProgram *pp = L.progs;
-    const GrnDataset *dp = L.datasets;
-    const int *op = L.order;
-    void *args[] = {&L, &pp, &dp, &op};
-    HIP_TRY(hipLaunchKernel(kc.k->fn, dim3(grid), dim3(kc.block), args, kc.lds, st));
-    ++launches;
   }
-  HIP_TRY(hipEventRecord(c->ev1, st));
+
+  hipStream_t st = c->stream;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(c->d_progs.p, progs.data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_order.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_maxerr.p, maxerr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(c->d_batch.p, &batch, sizeof(GrnBatch), hipMemcpyHostToDevice, st));
+
+  // The launches: result clearing, then one persistent kernel per class.
+  auto issue = [&]() -> int {
+    HIP_TRY(grn_launch_clear_results((const GrnBatch *)c->d_batch.p, (GrnModelOut *)c->d_out.p, st));
+    for (Launch &x : plan) {
+      const GrnProgram *pp = x.L.progs;
+      const GrnDataset *dp = x.L.datasets;
+      const int *op = (const int *)c->d_order.p;
+      void *args[] = {&x.L, &pp, &dp, &op};
+      HIP_TRY(hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, st));
+    }
+    return 0;
+  };
+  bool replayed = false;
+  auto launch = [&]() -> int {
+  if (c->launch_mode == GRN_LAUNCH_GRAPH && n > 0) {
+    // the layout: everything a captured launch bakes in
+    std::vector<long> key = {(long)c->d_batch.p, (long)c->d_out.p, (long)c->d_progs.p, (long)c->d_order.p,
+                             (long)c->d_maxerr.p, (long)c->d_datasets.p, (long)c->d_scratch.p,
+                             (long)(fields_host ? c->d_fields.p : nullptr), field_stride, field_exp};
+    for (const Launch &x : plan) {
+      key.insert(key.end(), {(long)x.kc.k->fn, x.grid, x.kc.block, (long)x.kc.lds, x.kc.slot_floats,
+                             x.kc.tile.tw, x.kc.tile.th, x.kc.tile.sx, x.kc.tile.sy, x.kc.tile.halo});
+    }
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      if (c->graphs.size() >= 32) c->drop_graphs();
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc_issue = issue();
+      hipGraph_t g = nullptr;
+      const hipError_t e_end = hipStreamEndCapture(st, &g);
+      if (rc_issue) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc_issue;
+      }
+      if (e_end != hipSuccess) return fail(GRN_E_DEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(e_end));
+      hipGraphExec_t ge = nullptr;
+      const hipError_t e_inst = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e_inst != hipSuccess) return fail(GRN_E_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e_inst));
+      it = c->graphs.emplace(key, ge).first;
+    } else {
+      replayed = true;
+    }
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    HIP_TRY(hipGraphLaunch(it->second, st));
+    HIP_TRY(hipEventRecord(c->ev1, st));
+  } else {
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    const int rc_issue = issue();
+    if (rc_issue) return rc_issue;
+    HIP_TRY(hipEventRecord(c->ev1, st));
+  }
+  return 0;
+  };
+  if ((rc = launch())) {
+    (void)hipStreamSynchronize(st);  // the uploads read host buffers of this frame
+    return rc;
+  }
+  const int launches = (int)plan.size();
 
   std::vector<GrnModelOut> res(n);
   if (n > 0)
@@ -418,6 +488,8 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   c->stats.alg_flops = flops;
   c->stats.n_launches = launches;
   c->stats.n_models = n;
+  c->stats.launch_mode = c->launch_mode;
+  c->stats.graph_replayed = replayed ? 1 : 0;
   c->stats.host_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   return 0;
@@ -581,6 +653,14 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, doubl
   if (capacity < need) return fail(GRN_E_INVALID, "fields buffer holds %ld of %ld values", capacity, need);
   std::memcpy(fields, buf.data(), sizeof(double) * need);
   if (n_products) *n_products = g;
+  return 0;
+}
+
+int grn_set_launch_mode(void *ctx, int mode) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  if (mode != GRN_LAUNCH_EAGER && mode != GRN_LAUNCH_GRAPH) return fail(GRN_E_INVALID, "unknown launch mode %d", mode);
+  c->launch_mode = mode;
   return 0;
 }
 

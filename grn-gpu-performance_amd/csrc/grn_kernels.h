@@ -2,6 +2,10 @@
 #ifndef GRN_KERNELS_H
 #define GRN_KERNELS_H
 
+#include <hip/hip_runtime.h>
+
+#include "grn_program.h"
+
 struct GrnKernelEntry {
   int wc;     // compile-time grid width, 0 = any width, -1 = halo-tiled kernel
   int cpt;    // cells per thread (vertical strip length)
@@ -18,5 +22,8 @@ extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
+
+// Zeroes out[0 .. batch->n_total) on stream (grn_probe.hip).
+hipError_t grn_launch_clear_results(const GrnBatch *batch, GrnModelOut *out, hipStream_t stream);
 
 #endif

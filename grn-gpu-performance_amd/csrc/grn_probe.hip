@@ -1,12 +1,38 @@
-// grn_probe.hip -- a plain device-to-device copy of known size.  Profilers
-// read the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of this kernel to
-// calibrate the same counters on the simulation kernels.
+// grn_probe.hip -- small utility kernels:
+//  * the result-clearing kernel that runs before every batch's simulation
+//    launches (a kernel rather than a memset so that it can sit in a captured
+//    graph; the batch size comes from the device-side batch table);
+//  * a plain device-to-device copy of known size, whose HBM byte counters
+//    (FETCH_SIZE, WRITE_SIZE) calibrate the same counters on the simulation
+//    kernels.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <string>
 
 #include "grn.h"
+#include "grn_kernels.h"
+#include "grn_program.h"
+
+__global__ __launch_bounds__(256) void grn_clear_results_kernel(const GrnBatch *__restrict__ batch,
+                                                                GrnModelOut *__restrict__ out) {
+  const int n = batch->n_total;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    GrnModelOut z;
+    z.err = 0.0;
+    z.change = 0.0;
+    z.steps = 0;
+    z.exps = 0;
+    z.done = 0;
+    z.pad = 0;
+    out[i] = z;
+  }
+}
+
+hipError_t grn_launch_clear_results(const GrnBatch *batch, GrnModelOut *out, hipStream_t stream) {
+  hipLaunchKernelGGL(grn_clear_results_kernel, dim3(64), dim3(256), 0, stream, batch, out);
+  return hipGetLastError();
+}
 
 __global__ __launch_bounds__(256) void grn_copy_probe_kernel(const float4 *__restrict__ in,
                                                              float4 *__restrict__ out, size_t n) {

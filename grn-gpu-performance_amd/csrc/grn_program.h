@@ -74,11 +74,28 @@ struct GrnModelOut {
   int pad;
 };
 
+// One launch's work queue, in device memory so that a captured launch can be
+// replayed on a new batch: only this table and the program arrays change.
+struct GrnQueue {
+  int head;   // next queue position to take; zero at launch
+  int n;      // models in this queue
+  int begin;  // first slot of this queue in the work-order array
+  int pad;
+};
+
+#define GRN_MAX_QUEUES 64
+
+// Per-batch table uploaded before the launches.
+struct GrnBatch {
+  int n_total;  // models in the batch (results to clear)
+  int n_queues;
+  int pad[2];
+  GrnQueue q[GRN_MAX_QUEUES];
+};
+
 struct GrnLaunch {
-  const GrnProgram *progs;  // [n]
-  const int *order;         // work order: program indices, costliest first
-  int n;
-  int *counter;             // work-queue head, zero at launch
+  const GrnProgram *progs;  // [n_total]
+  GrnQueue *queue;          // this launch's queue; the work order comes as a kernel argument
   const GrnDataset *datasets;
   const double *max_err;    // [n]
   GrnModelOut *out;         // [n]

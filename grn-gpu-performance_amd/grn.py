@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libgrn_hip.so")
 FITNESS_REF = 0
 FITNESS_MSE = 1
 PATH_AUTO, PATH_LDS, PATH_TILED = 0, 1, 2
+LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 MAX_GENES = 16
 MAX_LINKS = 96
 
@@ -38,7 +39,7 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
     "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
-    "grn_get_stats", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
+    "grn_get_stats", "grn_set_launch_mode", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
@@ -79,7 +80,8 @@ class _Result(ctypes.Structure):
 class _Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
                 ("cell_steps", ctypes.c_double), ("alg_flops", ctypes.c_double),
-                ("n_launches", ctypes.c_int), ("n_models", ctypes.c_int)]
+                ("n_launches", ctypes.c_int), ("n_models", ctypes.c_int),
+                ("launch_mode", ctypes.c_int), ("graph_replayed", ctypes.c_int)]
 
 
 class _SearchParams(ctypes.Structure):
@@ -137,6 +139,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_simulate_fields.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, vp, ctypes.c_long, ip]
     lib.grn_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
+    lib.grn_set_launch_mode.argtypes = [vp, ctypes.c_int]
     lib.grn_ceil_s.argtypes = [ctypes.c_double, ctypes.c_int]
     lib.grn_ceil_s.restype = ctypes.c_double
     lib.grn_model_parse.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip]
@@ -320,6 +323,10 @@ class Context:
         _check(self._lib.grn_simulate_fields(self._h, dataset, ctypes.byref(cm), dt, experiment, path,
                                              buf.ctypes.data, cap, ctypes.byref(g)))
         return buf[: g.value * p.dim_x * p.dim_y].reshape(g.value, p.dim_y, p.dim_x).copy()
+
+    def set_launch_mode(self, mode: int):
+        """LAUNCH_EAGER or LAUNCH_GRAPH (capture once per kernel-class layout, then replay)."""
+        _check(self._lib.grn_set_launch_mode(self._h, mode))
 
     def stats(self) -> dict:
         s = _Stats()

@@ -108,3 +108,97 @@ def make_target(ctx, w: Workload) -> np.ndarray:
     n_in, n_tgt = w.params.n_input_morphogens, w.params.n_target_morphogens
     w.target = f[n_in:n_in + n_tgt][None].copy()
     return w.target
+
+
+def _link(r, t, K, n, is_and):
+    return (r, t, K, n, int(is_and))
+
+
+def hidden_stripe_spot() -> Model:
+    """The hidden 8-gene GRN of the multi-island workload.
+
+    Genes 0 and 1 hold an x- and a y-gradient (no production, no decay); genes 4..7 switch on where the
+    gradient passes 0.3 or 0.6 (steep Hill activations); gene 2 is on where
+    x lies between the two thresholds (a stripe: activated by 4 AND repressed
+    by 5) and gene 3 where both x and y do (a spot: the same in both
+    directions).  Genes 2 and 3 diffuse slightly, which blurs the edges.
+    """
+    genes = np.zeros(8, GENE_DTYPE)
+    genes["label"] = np.arange(8)
+    genes["lim"] = [0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0]
+    genes["deg"] = [0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0]  # the input planes stay fixed
+    genes["dif"] = [0.0, 0.0, 0.02, 0.02, 0.0, 0.0, 0.0, 0.0]
+    links = np.array([
+        _link(0, 4, 0.3, 6.0, False), _link(0, 5, 0.6, 6.0, False),
+        _link(1, 6, 0.3, 6.0, False), _link(1, 7, 0.6, 6.0, False),
+        _link(4, 2, 0.5, 4.0, True), _link(5, 2, 0.5, -4.0, True),
+        _link(4, 3, 0.5, 4.0, True), _link(5, 3, 0.5, -4.0, True),
+        _link(6, 3, 0.5, 4.0, True), _link(7, 3, 0.5, -4.0, True),
+    ], dtype=LINK_DTYPE)
+    return Model(genes, links)
+
+
+def dt_rule(model: Model, scale: float = 0.2, cap: float = 1.0) -> float:
+    """dt = scale * dx^2 / max(D) with dx = 1, at most cap (cap when no gene diffuses)."""
+    d = float(np.max(model.genes["dif"])) if len(model.genes) else 0.0
+    return min(scale / d, cap) if d > 0 else cap
+
+
+SIDES = (32, 48, 64, 96, 128)
+
+
+@dataclass
+class IslandWorkload:
+    name: str
+    sides: list              # grid side of each island
+    params: list             # SimParams per island
+    init: list               # [1][2][s][s] per island
+    target: list             # [1][2][s][s] per island (filled by make_island_targets)
+    hidden: Model
+    search: dict             # keyword arguments of grn.Search
+
+
+def config2(seed: int = 0, n_islands: int = 16, island_size: int = 1024, steps: int = 3000,
+            sides=None) -> IslandWorkload:
+    """16 islands x 1024 models, 2-12 genes (4 structural), 1-40 links, grid
+    side per island from {32, 48, 64, 96, 128} (every side used at least once
+    when there are 5 or more islands), 3000 steps, dt = 0.2 dx^2 / max(D)
+    capped at 1.  Targets: stripe and spot of the hidden 8-gene GRN."""
+    rng = np.random.default_rng(seed)
+    if sides is None:
+        sides = list(SIDES[:min(n_islands, len(SIDES))])
+        sides += [int(s) for s in rng.choice(SIDES, n_islands - len(sides))]
+    params, init = [], []
+    for s in sides:
+        params.append(SimParams(s, s, 2, 2, steps, 1.0, fitness_kind=FITNESS_MSE))
+        init.append(np.stack([gradient_x(s, s), gradient_y(s, s)])[None])
+    search = dict(n_islands=n_islands, island_size=island_size, n_morphogens=4, n_max_products=12,
+                  n_max_links=40, init_products_min=4, init_products_max=12,
+                  migration=1, migration_period=10, migration_fraction=0.05,
+                  lim_lo=0.0, lim_hi=1.0, deg_lo=0.01, deg_hi=1.0, dif_prob=0.5, dif_lo=0.0,
+                  dif_hi=0.1, k_lo=0.05, k_hi=1.0, n_lo=1.0, n_hi=4.0,
+                  dt_rule=1, dt_scale=0.2, dt_cap=1.0, seed=seed + 1)
+    return IslandWorkload(f"config2: {n_islands} islands x {island_size}, sides {sorted(set(sides))}, "
+                          f"{steps} steps", sides, params, init, [None] * n_islands,
+                          hidden_stripe_spot(), search)
+
+
+def make_island_targets(ctx, w: IslandWorkload) -> list:
+    """Simulates the hidden GRN on every island's grid (at its own dt rule)."""
+    dt = dt_rule(w.hidden, w.search["dt_scale"], w.search["dt_cap"])
+    cache = {}
+    for i, (p, init) in enumerate(zip(w.params, w.init)):
+        if p.dim_x not in cache:
+            dummy = np.zeros((1, 2, p.dim_y, p.dim_x))
+            ds = ctx.add_dataset(p, init, dummy)
+            f = ctx.simulate_fields(w.hidden, dataset=ds, dt=dt)
+            cache[p.dim_x] = f[2:4][None].copy()
+        w.target[i] = cache[p.dim_x]
+    return w.target
+
+
+def add_island_datasets(ctx, w: IslandWorkload) -> list:
+    """Uploads one dataset per island; returns the dataset ids."""
+    if any(t is None for t in w.target):
+        make_island_targets(ctx, w)
+    return [ctx.add_dataset(p, i, t) for p, i, t in zip(w.params, w.init, w.target)]

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define GRN_ABI_VERSION 1
+#define GRN_ABI_VERSION 2
 
 /* Capacity of the device program (compile-time limits of the kernels). */
 #define GRN_MAX_GENES 16  /* genes in use per model */
@@ -120,9 +120,16 @@ typedef struct {
   double host_ms;         /* wall time of the last evaluate call */
   double cell_steps;      /* sum over models of W*H*steps executed */
   double alg_flops;       /* sum over models of cell-steps * F(model) */
-  int n_launches;         /* kernel launches issued by the last call */
+  int n_launches;         /* simulation launches of the last call (one per kernel class) */
   int n_models;
+  int launch_mode;        /* grn_launch_mode the last call ran with */
+  int graph_replayed;     /* 1: the last call replayed a graph captured earlier */
 } grn_stats;
+
+typedef enum {
+  GRN_LAUNCH_EAGER = 0, /* launches issued one by one on the context's stream */
+  GRN_LAUNCH_GRAPH = 1  /* launches captured once per kernel-class layout, then replayed */
+} grn_launch_mode;
 
 /* Library ABI version (GRN_ABI_VERSION). */
 int grn_abi_version(void);
@@ -165,6 +172,14 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
 int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model,
                         double dt_override, int experiment, int path,
                         double *fields, long capacity, int *n_products);
+
+/* No reference counterpart: how grn_evaluate_batch (and the search calls
+ * that use it) issue a batch's launches.  GRN_LAUNCH_GRAPH captures the
+ * result clearing and the persistent launches in a HIP graph the first time
+ * a layout of kernel classes is seen and replays it for later batches of the
+ * same layout; every launch reads its work queue from a device table that is
+ * uploaded per batch, so results are identical to GRN_LAUNCH_EAGER. */
+int grn_set_launch_mode(void *ctx, int mode);
 
 /* Timing and work counters of the last evaluate call on this context. */
 int grn_get_stats(void *ctx, grn_stats *out);

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     lib = grn.load_library()
-    assert lib.grn_abi_version() == 1
+    assert lib.grn_abi_version() == 2
     assert lib.grn_last_error() is not None
 
 
@@ -102,3 +102,41 @@ def test_compile_rejects_missing_product_and_capacity():
     with pytest.raises(grn.GrnError) as e:
         grn.Model(genes, links).info(1)
     assert e.value.code == -3
+
+
+def _c_layout(tmp_path, structs):
+    """sizeof and field offsets of the header's structs, as gcc lays them out."""
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "grn.h"', "int main(void) {"]
+    for name, fields in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for f in fields:
+            lines.append(f'  printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    return {k: int(v) for k, v in (line.split() for line in out.splitlines())}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    import ctypes
+    mirrors = {"grn_sim_params": grn._SimParams, "grn_model": grn._Model, "grn_result": grn._Result,
+               "grn_stats": grn._Stats, "grn_search_params": grn._SearchParams,
+               "grn_generation_stats": grn._GenerationStats}
+    dtypes = {"grn_gene": grn.GENE_DTYPE, "grn_link": grn.LINK_DTYPE}
+    structs = {k: [f for f, _ in v._fields_] for k, v in mirrors.items()}
+    structs.update({k: list(v.names) for k, v in dtypes.items()})
+    lay = _c_layout(tmp_path, structs)
+    for name, cls in mirrors.items():
+        assert lay[name] == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert lay[f"{name}.{f}"] == getattr(cls, f).offset, f"{name}.{f}"
+    for name, dt in dtypes.items():
+        assert lay[name] == dt.itemsize, name
+        for f in dt.names:
+            assert lay[f"{name}.{f}"] == dt.fields[f][1], f"{name}.{f}"
+    assert grn.RESULT_DTYPE.itemsize == lay["grn_result"]
+    for f in grn.RESULT_DTYPE.names:
+        assert grn.RESULT_DTYPE.fields[f][1] == lay[f"grn_result.{f}"]

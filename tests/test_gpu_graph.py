@@ -1,0 +1,88 @@
+"""Graph-captured batches (grn_set_launch_mode(GRN_LAUNCH_GRAPH)).
+
+A batch's launches (result clearing plus one persistent kernel per class)
+are captured once per layout and replayed; each launch reads its work queue
+from a device table uploaded per batch.  Results must equal the eager
+launches' bit for bit on every replay, with eager work on the same context
+and other batches in between, and a search driven through captured
+generations must follow the eager search exactly (same seeds).
+"""
+import numpy as np
+import pytest
+
+import grn
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    print("\n[gpu] creating context on device 0", flush=True)
+    c = grn.Context(0)
+    yield c
+    c.set_launch_mode(grn.LAUNCH_EAGER)
+    c.close()
+
+
+def mixed_batch(ctx, seed=0, steps=200):
+    """Models on three grids (two on-chip widths and one halo-tiled side)."""
+    rng = np.random.default_rng(seed)
+    ids, models = [], []
+    for s in (32, 48, 96):
+        p = grn.SimParams(s, s, 2, 1, steps, 0.05, fitness_kind=grn.FITNESS_MSE)
+        init = np.stack([workloads.gradient_x(s, s), workloads.gradient_y(s, s)])[None]
+        target = rng.uniform(0, 1, (1, 1, s, s))
+        ds = ctx.add_dataset(p, init, target)
+        ms = workloads._population(rng, 24, (3, 12), (1, 30), 3)
+        ids += [ds] * len(ms)
+        models += ms
+    return ids, models
+
+
+def test_graph_replays_equal_eager(ctx):
+    ids, models = mixed_batch(ctx)
+    ids2, models2 = mixed_batch(ctx, seed=1)
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    want = ctx.evaluate_batch(models, datasets=ids)
+    want2 = ctx.evaluate_batch(models2, datasets=ids2)
+    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+    for rep in range(4):
+        print(f"[gpu] graph replay {rep}", flush=True)
+        got = ctx.evaluate_batch(models, datasets=ids)
+        st = ctx.stats()
+        assert st["launch_mode"] == grn.LAUNCH_GRAPH and st["n_launches"] >= 3
+        assert np.array_equal(got, want)
+        if rep > 0:
+            assert st["graph_replayed"] == 1
+        # eager work and a different batch between replays
+        ctx.set_launch_mode(grn.LAUNCH_EAGER)
+        assert np.array_equal(ctx.evaluate_batch(models2, datasets=ids2), want2)
+        ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+        assert np.array_equal(ctx.evaluate_batch(models2, datasets=ids2), want2)
+    # a smaller batch (its own layout) keeps every model's result
+    sub = [0, 1, 30, 31, 60, 61]
+    got = ctx.evaluate_batch([models[i] for i in sub], datasets=[ids[i] for i in sub])
+    assert np.array_equal(got, want[sub])
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+
+
+def test_search_through_captured_generations_equals_eager(ctx):
+    w = workloads.config2(seed=3, n_islands=3, island_size=48, steps=200, sides=[32, 48, 96])
+    ds = workloads.add_island_datasets(ctx, w)
+    params = dict(w.search, migration_period=2)
+    a = grn.Search(island_datasets=ds, **params)
+    b = grn.Search(island_datasets=ds, **params)
+    replays = 0
+    for gen in range(6):
+        print(f"[gpu] search generation {gen}", flush=True)
+        ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+        sa = a.run_generation(ctx)
+        replays += ctx.stats()["graph_replayed"]
+        ctx.set_launch_mode(grn.LAUNCH_EAGER)
+        sb = b.run_generation(ctx)
+        assert sa == sb
+        pa, pb = a.pending(), b.pending()
+        assert [m.to_text(17) for m in pa[0]] == [m.to_text(17) for m in pb[0]]
+    assert a.best()[1] == b.best()[1]
+    print(f"[gpu] graph replays: {replays}", flush=True)
