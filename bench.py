@@ -63,6 +63,65 @@ def cpu_baseline(w, budget_s=15.0):
                       f"{threads} threads, {dt:.1f} s"}
 
 
+def bench_islands(args):
+    """Multi-island evolution on one GPU: every generation (simulation and
+    fitness of all islands' children in one batch of persistent launches
+    under a captured HIP graph, then selection and ring migration) is one
+    step."""
+    import grn
+    import workloads
+
+    w = workloads.config2(seed=args.seed, n_islands=args.islands, island_size=args.island_size)
+    ctx = grn.Context(0)
+    ds = workloads.add_island_datasets(ctx, w)
+    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+    search = grn.Search(island_datasets=ds, **w.search)
+    log(f"[bench] {w.name}: warmup x{args.warmup}")
+    for _ in range(args.warmup):
+        search.run_generation(ctx)
+    log(f"[bench] timing x{args.steps} generations")
+    cell_steps, alg_flops, kernel_ms, replays, gens = 0.0, 0.0, [], 0, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = search.run_generation(ctx)
+        st = ctx.stats()
+        cell_steps += st["cell_steps"]
+        alg_flops += st["alg_flops"]
+        kernel_ms.append(st["kernel_ms"])
+        replays += st["graph_replayed"]
+        gens.append(g)
+    wall = time.perf_counter() - t0
+    k = float(np.sum(kernel_ms)) / 1e3
+    achieved = alg_flops / k / 1e12
+    out = {
+        "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
+        "value": cell_steps / wall,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "multi-island evolution, config 2: " + w.name, "islands": args.islands,
+                   "island_size": args.island_size, "sides": w.sides, "sim_steps": w.params[0].num_sim_steps,
+                   "parallelism": "one graph-captured batch of persistent launches per generation"},
+        "kernel_cell_steps_per_s": cell_steps / k,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel_ms": float(np.mean(kernel_ms))},
+        "graph_replays": replays,
+        "launches_per_generation": ctx.stats()["n_launches"],
+        "best_fitness": gens[-1]["min_fit"],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +131,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["config1", "config2"], default="config1",
+                    help="config1: one batch of 4096 models on 64x64 (the headline line); config2: "
+                         "multi-island evolution generations on mixed grid sides")
+    ap.add_argument("--islands", type=int, default=16)
+    ap.add_argument("--island-size", type=int, default=1024)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device with the library's "
                          "copy kernel (known size, to calibrate FETCH_SIZE / WRITE_SIZE counters)")
@@ -80,6 +144,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(models shard across GPUs by island, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "config2":
+        return bench_islands(args)
 
     import grn
     import workloads
