@@ -213,19 +213,7 @@ int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int 
 Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
 
 int to_model(const grn_model *gm, grn::Model *m) {
-  if (!gm || gm->n_genes < 0 || gm->n_links < 0 || (gm->n_genes && !gm->genes) ||
-      (gm->n_links && !gm->links))
-    return fail(GRN_E_INVALID, "invalid grn_model");
-  m->genes.resize(gm->n_genes);
-  for (int i = 0; i < gm->n_genes; ++i) {
-    const grn_gene &g = gm->genes[i];
-    m->genes[i] = grn::Gene{g.label, g.init, g.lim, g.deg, g.dif};
-  }
-  m->links.resize(gm->n_links);
-  for (int i = 0; i < gm->n_links; ++i) {
-    const grn_link &l = gm->links[i];
-    m->links[i] = grn::Link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and != 0};
-  }
+  if (!grn::model_from_abi(gm, m)) return fail(GRN_E_INVALID, "invalid grn_model");
   return 0;
 }
 

@@ -1,0 +1,284 @@
+// grn_mlp.hip -- kernels of the surrogate network that pre-screens offspring:
+// a 512-wide MLP (four GELU hidden layers, one scalar output) trained online
+// on (model encoding, log-fitness) pairs produced by the simulator.
+//
+// All hidden layers are 512x512 GEMMs in BF16 with FP32 accumulation on the
+// gfx950 matrix cores (v_mfma_f32_16x16x32_bf16).  Every GEMM is in "NT" form,
+//   C[m][n] = sum_k A[m][k] * Bt[n][k],
+// with both operands K-contiguous, so that a tile's rows load as 16-byte
+// vectors into LDS and each lane's MFMA fragment (8 consecutive k) is one
+// 16-byte LDS read.  Layouts that the backward pass needs in the other
+// orientation are written by the producing epilogue (H and H^T, dZ and
+// dZ^T; W and W^T by the optimizer), so no separate transpose kernel runs.
+//
+// Fragment layout of v_mfma_f32_16x16x32_bf16 (lane l of 64):
+//   A: row l%16, k = 8*(l/16) + e (e = 0..7)
+//   B: column l%16, k = 8*(l/16) + e
+//   D: rows 4*(l/16) + r (r = 0..3), column l%16
+#include <hip/hip_runtime.h>
+
+#include "grn_mlp.h"
+
+namespace grn_mlp {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int LDS_STRIDE = 40;  // bf16 per LDS tile row: 32 + 8 padding (80 bytes)
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+__device__ __forceinline__ float gelu_grad(float z) {
+  // d/dz [z Phi(z)] = Phi(z) + z phi(z)
+  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * z * z);
+  return cdf + z * pdf;
+}
+
+// C = A . Bt^T for M % 128 == 0, N % 128 == 0, K % 32 == 0 (checked by the
+// host).  256 threads: 4 waves in a 2 x 2 arrangement, each wave a 64 x 64
+// block of sixteen 16 x 16 accumulators.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[BM * LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LDS_STRIDE];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int K = g.K;
+
+  // each thread stages 16 bf16 (two 16-byte vectors) of one A row and one Bt row
+  const int lrow = t >> 1, lhalf = t & 1;
+  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + lrow) * K + 16 * lhalf);
+  const uint4 *pb = (const uint4 *)(g.Bt + (size_t)(n0 + lrow) * K + 16 * lhalf);
+  uint4 ra0 = pa[0], ra1 = pa[1], rb0 = pb[0], rb1 = pb[1];
+
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  uint4 *sa = (uint4 *)(As + lrow * LDS_STRIDE + 16 * lhalf);
+  uint4 *sb = (uint4 *)(Bs + lrow * LDS_STRIDE + 16 * lhalf);
+  const int frow = lane & 15, fk = 8 * (lane >> 4);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();  // the previous tile's fragment reads are done
+    sa[0] = ra0;
+    sa[1] = ra1;
+    sb[0] = rb0;
+    sb[1] = rb1;
+    __syncthreads();
+    if (k0 + BK < K) {  // prefetch the next K slice while this one multiplies
+      pa += 4;
+      pb += 4;
+      ra0 = pa[0];
+      ra1 = pa[1];
+      rb0 = pb[0];
+      rb1 = pb[1];
+    }
+    bf16x8 af[4], bfr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8 *)(As + (64 * wm + 16 * i + frow) * LDS_STRIDE + fk);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8 *)(Bs + (64 * wn + 16 * j + frow) * LDS_STRIDE + fk);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+  }
+
+  const int M = g.M, N = g.N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int mb = m0 + 64 * wm + 16 * i + 4 * (lane >> 4);  // first of this lane's 4 rows
+      const int n = n0 + 64 * wn + 16 * j + frow;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const size_t o = (size_t)(mb + r) * N + n;
+        float x = acc[i][j][r];
+        if constexpr (EPI == EPI_FWD) {
+          x += g.bias[n];
+          if (g.C) g.C[o] = x;  // pre-activation, kept for the backward pass
+          x = gelu(x);
+          g.H[o] = (bf16)x;
+        } else if constexpr (EPI == EPI_BWD) {
+          x *= gelu_grad(g.Zprev[o]);
+          g.H[o] = (bf16)x;
+        } else {
+          g.C[o] = x;
+        }
+        v[r] = x;
+      }
+      if constexpr (EPI != EPI_F32) {
+        if (g.Ht) {  // the transposed copy: 4 consecutive rows -> one 8-byte store
+          bf16x4 q = {(bf16)v[0], (bf16)v[1], (bf16)v[2], (bf16)v[3]};
+          *(bf16x4 *)(g.Ht + (size_t)n * M + mb) = q;
+        }
+      }
+    }
+}
+
+// Output layer, one wave per row: y[b] = H[b] . w + b0 (512 inputs).
+__global__ __launch_bounds__(256) void out_forward_kernel(const bf16 *__restrict__ H, const bf16 *__restrict__ w,
+                                                          const float *__restrict__ b0, float *__restrict__ y,
+                                                          int rows) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const bf16x8 h = *(const bf16x8 *)(H + (size_t)row * MLP_WIDTH + 8 * lane);
+  const bf16x8 ww = *(const bf16x8 *)(w + 8 * lane);
+  float s = 0.f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) s += (float)h[e] * (float)ww[e];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) y[row] = s + b0[0];
+}
+
+// Loss gradient at the output and the last hidden layer's dZ:
+//   dy[b] = mask[b] * 2 (y[b] - t[b]) / n_valid,
+//   dZ[b][k] = dy[b] * w[k] * gelu'(Z[b][k]), written as dZ and dZ^T.
+__global__ __launch_bounds__(256) void out_backward_kernel(const float *__restrict__ y, const float *__restrict__ tgt,
+                                                           const float *__restrict__ mask, const bf16 *__restrict__ w,
+                                                           const float *__restrict__ Z, float inv_n, int rows,
+                                                           float *__restrict__ dy, float *__restrict__ sq,
+                                                           bf16 *__restrict__ dZ, bf16 *__restrict__ dZt) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const float e = mask[row] * (y[row] - tgt[row]);
+  const float d = 2.f * e * inv_n;
+  if (lane == 0) {
+    dy[row] = d;
+    sq[row] = e * e;
+  }
+  const bf16x8 ww = *(const bf16x8 *)(w + 8 * lane);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int col = 8 * lane + k;
+    const float v = d * (float)ww[k] * gelu_grad(Z[(size_t)row * MLP_WIDTH + col]);
+    dZ[(size_t)row * MLP_WIDTH + col] = (bf16)v;
+    dZt[(size_t)col * rows + row] = (bf16)v;
+  }
+}
+
+// out[r] = sum_c X[r][c] * (v ? v[c] : 1) over a bf16 matrix with `cols`
+// columns, one workgroup per row, fixed summation order (deterministic).
+__global__ __launch_bounds__(256) void row_dot_kernel(const bf16 *__restrict__ X, const float *__restrict__ v,
+                                                      int cols, float *__restrict__ out) {
+  __shared__ float part[4];
+  const int r = blockIdx.x;
+  float s = 0.f;
+  for (int c = threadIdx.x; c < cols; c += 256) s += (float)X[(size_t)r * cols + c] * (v ? v[c] : 1.f);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[r] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// Sum of a float vector into out[0] (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void sum_kernel(const float *__restrict__ x, int n, float *__restrict__ out) {
+  __shared__ float part[4];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// Adam over the flat FP32 parameter vector, then the BF16 copies the GEMMs
+// read: every hidden weight matrix as W^T (out x in, the forward operand) and
+// W (in x out, the backward operand), and the output weights.
+__global__ __launch_bounds__(256) void adam_kernel(MlpAdam a) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const float g = a.grad[i];
+  const float m = a.beta1 * a.m[i] + (1.f - a.beta1) * g;
+  const float v = a.beta2 * a.v[i] + (1.f - a.beta2) * g * g;
+  a.m[i] = m;
+  a.v[i] = v;
+  const float p = a.p[i] - a.lr_t * m / (sqrtf(v) + a.eps_t);
+  a.p[i] = p;
+  const long mat = (long)MLP_HIDDEN_LAYERS * MLP_WIDTH * MLP_WIDTH;
+  if (i < mat) {
+    const long l = i / (MLP_WIDTH * MLP_WIDTH), e = i % (MLP_WIDTH * MLP_WIDTH);
+    const long o = e / MLP_WIDTH, in = e % MLP_WIDTH;
+    a.wt[i] = (bf16)p;
+    a.w[l * MLP_WIDTH * MLP_WIDTH + in * MLP_WIDTH + o] = (bf16)p;
+  } else if (i >= a.out_w_offset && i < a.out_w_offset + MLP_WIDTH) {
+    a.w_out[i - a.out_w_offset] = (bf16)p;
+  }
+}
+
+// Training batch: rows idx[b] of the sample store (features and targets).
+// Targets are standardised: t = (target - mu) * inv_sigma.
+__global__ __launch_bounds__(256) void gather_kernel(const bf16 *__restrict__ feat, const float *__restrict__ target,
+                                                     const int *__restrict__ idx, int rows, float mu, float inv_sigma,
+                                                     bf16 *__restrict__ X, bf16 *__restrict__ Xt,
+                                                     float *__restrict__ t) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const int src = idx[row];
+  const bf16x8 x = *(const bf16x8 *)(feat + (size_t)src * MLP_WIDTH + 8 * lane);
+  *(bf16x8 *)(X + (size_t)row * MLP_WIDTH + 8 * lane) = x;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) Xt[(size_t)(8 * lane + k) * rows + row] = x[k];
+  if (lane == 0) t[row] = (target[src] - mu) * inv_sigma;
+}
+
+}  // namespace grn_mlp
+
+using namespace grn_mlp;
+
+hipError_t grn_mlp_gemm(int epi, const MlpGemm &g, hipStream_t s) {
+  const dim3 grid(g.N / BN, g.M / BM), block(256);
+  if (epi == EPI_FWD)
+    hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, block, 0, s, g);
+  else if (epi == EPI_BWD)
+    hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, block, 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_out_forward(const __bf16 *H, const __bf16 *w, const float *b0, float *y, int rows, hipStream_t s) {
+  hipLaunchKernelGGL(out_forward_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, H, w, b0, y, rows);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_out_backward(const float *y, const float *t, const float *mask, const __bf16 *w, const float *Z,
+                                float inv_n, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(out_backward_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, y, t, mask, w, Z, inv_n, rows, dy,
+                     sq, dZ, dZt);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_row_dot(const __bf16 *X, const float *v, int rows, int cols, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(row_dot_kernel, dim3(rows), dim3(256), 0, s, X, v, cols, out);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_sum(const float *x, int n, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, x, n, out);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_adam(const MlpAdam &a, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t grn_mlp_gather(const __bf16 *feat, const float *target, const int *idx, int rows, float mu,
+                          float inv_sigma, __bf16 *X, __bf16 *Xt, float *t, hipStream_t s) {
+  hipLaunchKernelGGL(gather_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, feat, target, idx, rows, mu, inv_sigma, X,
+                     Xt, t);
+  return hipGetLastError();
+}

@@ -263,4 +263,20 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
   return 0;
 }
 
+bool model_from_abi(const grn_model *gm, Model *m) {
+  if (!gm || gm->n_genes < 0 || gm->n_links < 0 || (gm->n_genes && !gm->genes) || (gm->n_links && !gm->links))
+    return false;
+  m->genes.resize(gm->n_genes);
+  for (int i = 0; i < gm->n_genes; ++i) {
+    const grn_gene &g = gm->genes[i];
+    m->genes[i] = Gene{g.label, g.init, g.lim, g.deg, g.dif};
+  }
+  m->links.resize(gm->n_links);
+  for (int i = 0; i < gm->n_links; ++i) {
+    const grn_link &l = gm->links[i];
+    m->links[i] = Link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and != 0};
+  }
+  return true;
+}
+
 }  // namespace grn

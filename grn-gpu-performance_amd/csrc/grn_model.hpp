@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "grn.h"
 #include "grn_program.h"
 
 namespace grn {
@@ -46,6 +47,10 @@ struct Model {
   // Returns false and sets *err on malformed text.
   static bool parse(const std::string &text, Model *out, std::string *err);
 };
+
+// The C ABI's grn_model -> Model; false when the struct is malformed
+// (negative counts, null arrays with nonzero counts).
+bool model_from_abi(const grn_model *gm, Model *out);
 
 // Work counts of a compiled model, for ordering and the roofline.
 struct ProgramInfo {

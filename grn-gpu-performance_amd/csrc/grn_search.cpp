@@ -3,6 +3,7 @@
 #include "grn_search.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -11,7 +12,43 @@
 #include <set>
 #include <string>
 
+#include "grn_surrogate.hpp"
+
 namespace grn {
+
+// Spearman rank correlation (average ranks for ties); 0 for fewer than 2 points.
+double spearman(const std::vector<double> &a, const std::vector<double> &b) {
+  const size_t n = a.size();
+  if (n < 2) return 0;
+  auto ranks = [n](const std::vector<double> &v) {
+    std::vector<int> o(n);
+    std::iota(o.begin(), o.end(), 0);
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return v[x] < v[y]; });
+    std::vector<double> r(n);
+    for (size_t i = 0; i < n;) {
+      size_t j = i;
+      while (j + 1 < n && v[o[j + 1]] == v[o[i]]) ++j;
+      for (size_t k = i; k <= j; ++k) r[o[k]] = 0.5 * (double)(i + j);
+      i = j + 1;
+    }
+    return r;
+  };
+  const std::vector<double> ra = ranks(a), rb = ranks(b);
+  double ma = 0, mb = 0;
+  for (size_t i = 0; i < n; ++i) {
+    ma += ra[i];
+    mb += rb[i];
+  }
+  ma /= n;
+  mb /= n;
+  double sab = 0, saa = 0, sbb = 0;
+  for (size_t i = 0; i < n; ++i) {
+    sab += (ra[i] - ma) * (rb[i] - mb);
+    saa += (ra[i] - ma) * (ra[i] - ma);
+    sbb += (rb[i] - mb) * (rb[i] - mb);
+  }
+  return saa > 0 && sbb > 0 ? sab / std::sqrt(saa * sbb) : 0;
+}
 
 // ---- RNG ---------------------------------------------------------------------
 
@@ -290,6 +327,7 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
   for (int i = 0; i <= p_.n_islands; ++i) rngs_.emplace_back(p_.seed * 0x100000001B3ULL + 0x9E37u * (uint64_t)(i + 1));
   islands_.resize(p_.n_islands);
   children_.resize(p_.n_islands);
+  cand_.resize(p_.n_islands);
   perm_.resize(p_.n_islands);
   for (int is = 0; is < p_.n_islands; ++is) {
     Rng &rng = rngs_[is];
@@ -324,32 +362,64 @@ void Search::prepare(Individual *ind, int island) {
   }
 }
 
+void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2) {
+  // one iteration of SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:70-97)
+  if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
+    cross(rng, a.model, b.model, &k1->model, &k2->model);
+  } else {
+    k1->model = a.model;
+    k2->model = b.model;
+  }
+  mutate(rng, &k1->model);
+  mutate(rng, &k2->model);
+  k1->parent_error = a.error;
+  k2->parent_error = b.error;
+  k1->dataset = a.dataset;
+  k2->dataset = b.dataset;
+  prepare(k1, 0);
+  prepare(k2, 0);
+}
+
 void Search::reproduce(int is) {
-  // SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:63-101)
+  // SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:63-101): shuffle
+  // the parents, breed consecutive pairs; child i belongs to parent perm[i]
   Rng &rng = rngs_[is];
   std::vector<int> &perm = perm_[is];
   for (int i = (int)perm.size() - 1; i > 0; --i) std::swap(perm[i], perm[rng.below(i + 1)]);  // MathAlgo::shuffle
-  std::vector<Individual> &pop = islands_[is];
+  const std::vector<Individual> &pop = islands_[is];
+  const int n = (int)pop.size();
   std::vector<Individual> &kids = children_[is];
-  kids.assign(pop.size(), Individual());
-  for (size_t i = 0; i + 1 < perm.size(); i += 2) {
-    const Individual &a = pop[perm[i]], &b = pop[perm[i + 1]];
-    Individual &k1 = kids[i], &k2 = kids[i + 1];
-    if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
-      cross(rng, a.model, b.model, &k1.model, &k2.model);
-    } else {
-      k1.model = a.model;
-      k2.model = b.model;
-    }
-    mutate(rng, &k1.model);
-    mutate(rng, &k2.model);
-    k1.parent_error = a.error;
-    k2.parent_error = b.error;
-    k1.dataset = a.dataset;
-    k2.dataset = b.dataset;
-    prepare(&k1, is);
-    prepare(&k2, is);
+  kids.assign(n, Individual());
+  if (oversample_ <= 1) {
+    for (int i = 0; i + 1 < n; i += 2) breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1]);
+    return;
   }
+  // k candidate pairs per parent pair, all with the same parents
+  std::vector<Individual> &cand = cand_[is];
+  cand.assign((size_t)oversample_ * n, Individual());
+  for (int r = 0; r < oversample_; ++r)
+    for (int i = 0; i + 1 < n; i += 2)
+      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1]);
+}
+
+void Search::screen(const double *scores) {
+  size_t off = 0;
+  pending_.clear();
+  for (int is = 0; is < p_.n_islands; ++is) {
+    std::vector<Individual> &cand = cand_[is];
+    std::vector<Individual> &kids = children_[is];
+    const int n = (int)kids.size();
+    for (int i = 0; i < n; ++i) {
+      int best = 0;
+      for (int r = 1; r < oversample_; ++r)
+        if (scores[off + (size_t)r * n + i] < scores[off + (size_t)best * n + i]) best = r;
+      kids[i] = std::move(cand[(size_t)best * n + i]);
+      pending_.push_back(&kids[i]);
+    }
+    off += cand.size();
+    cand.clear();
+  }
+  cand_ptrs_.clear();
 }
 
 void Search::choose_next_generation(int is, std::vector<Individual *> *winners) {
@@ -480,9 +550,13 @@ void Search::advance(const double *errors) {
   stats_.n_evaluated = n_evaluated;
   if (p_.migration_period > 0 && generation_ % p_.migration_period == 0) migrate();
   pending_.clear();
+  cand_ptrs_.clear();
   for (int is = 0; is < p_.n_islands; ++is) {
     reproduce(is);
-    for (Individual &k : children_[is]) pending_.push_back(&k);
+    if (oversample_ <= 1)
+      for (Individual &k : children_[is]) pending_.push_back(&k);
+    else
+      for (Individual &k : cand_[is]) cand_ptrs_.push_back(&k);
   }
 }
 
@@ -492,33 +566,29 @@ void Search::advance(const double *errors) {
 
 namespace {
 
-// A search plus the flat arrays grn_search_pending hands out.
-struct SearchHandle {
-  std::unique_ptr<grn::Search> s;
+// Flat C views of a list of individuals.
+struct FlatModels {
   std::vector<grn_model> models;
   std::vector<grn_gene> genes;
   std::vector<grn_link> links;
   std::vector<int> datasets;
   std::vector<double> max_errors, dts;
-  std::vector<grn_result> results;
-  std::vector<double> errors;
 
-  void flatten() {
-    const auto &pend = s->pending();
+  void build(const std::vector<grn::Individual *> &list) {
     size_t ng = 0, nl = 0;
-    for (const grn::Individual *x : pend) {
+    for (const grn::Individual *x : list) {
       ng += x->model.genes.size();
       nl += x->model.links.size();
     }
     genes.resize(ng);
     links.resize(nl);
-    models.resize(pend.size());
-    datasets.resize(pend.size());
-    max_errors.resize(pend.size());
-    dts.resize(pend.size());
+    models.resize(list.size());
+    datasets.resize(list.size());
+    max_errors.resize(list.size());
+    dts.resize(list.size());
     size_t gi = 0, li = 0;
-    for (size_t i = 0; i < pend.size(); ++i) {
-      const grn::Individual &x = *pend[i];
+    for (size_t i = 0; i < list.size(); ++i) {
+      const grn::Individual &x = *list[i];
       grn_model &m = models[i];
       m.n_genes = (int)x.model.genes.size();
       m.genes = genes.data() + gi;
@@ -531,6 +601,19 @@ struct SearchHandle {
       max_errors[i] = x.parent_error;
       dts[i] = x.dt;
     }
+  }
+};
+
+// A search plus the flat arrays grn_search_pending / _candidates hand out.
+struct SearchHandle {
+  std::unique_ptr<grn::Search> s;
+  FlatModels pend, cand;
+  std::vector<grn_result> results;
+  std::vector<double> errors;
+
+  void flatten() {
+    pend.build(s->pending());
+    cand.build(s->candidates());
   }
 };
 
@@ -599,18 +682,20 @@ int grn_search_pending(void *search, int *n, const grn_model **models, const int
                        const double **max_errors, const double **dt) {
   SearchHandle *h = as_search(search);
   if (!h || !n) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  *n = (int)h->models.size();
-  if (models) *models = h->models.data();
-  if (datasets) *datasets = h->datasets.data();
-  if (max_errors) *max_errors = h->max_errors.data();
-  if (dt) *dt = h->dts.data();
+  *n = (int)h->pend.models.size();
+  if (models) *models = h->pend.models.data();
+  if (datasets) *datasets = h->pend.datasets.data();
+  if (max_errors) *max_errors = h->pend.max_errors.data();
+  if (dt) *dt = h->pend.dts.data();
   return 0;
 }
 
 int grn_search_advance(void *search, const double *errors, grn_generation_stats *stats) {
   SearchHandle *h = as_search(search);
-  if (!h || (!errors && !h->models.empty())) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  for (size_t i = 0; i < h->models.size(); ++i)
+  if (!h || (!errors && !h->pend.models.empty())) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  if (h->pend.models.empty() && !h->cand.models.empty())
+    return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
+  for (size_t i = 0; i < h->pend.models.size(); ++i)
     if (!(errors[i] >= 0) || std::isinf(errors[i]))
       return grn::set_last_error(GRN_E_INVALID, "error %zu is %g; errors must be finite and >= 0", i, errors[i]);
   h->s->advance(errors);
@@ -622,14 +707,127 @@ int grn_search_advance(void *search, const double *errors, grn_generation_stats 
 int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats) {
   SearchHandle *h = as_search(search);
   if (!h || !ctx) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  const int n = (int)h->models.size();
+  if (h->pend.models.empty() && !h->cand.models.empty())
+    return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
+  const int n = (int)h->pend.models.size();
   h->results.resize(n);
   h->errors.resize(n);
-  int rc = grn_evaluate_batch(ctx, n, h->datasets.data(), h->models.data(), h->max_errors.data(), h->dts.data(),
-                              path, h->results.data());
+  int rc = grn_evaluate_batch(ctx, n, h->pend.datasets.data(), h->pend.models.data(), h->pend.max_errors.data(),
+                              h->pend.dts.data(), path, h->results.data());
   if (rc) return rc;
   for (int i = 0; i < n; ++i) h->errors[i] = h->results[i].fitness;
   return grn_search_advance(search, h->errors.data(), stats);
+}
+
+int grn_search_set_oversample(void *search, int k) {
+  SearchHandle *h = as_search(search);
+  if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
+  if (k < 1 || k > 64) return grn::set_last_error(GRN_E_INVALID, "oversample must lie in [1, 64]");
+  h->s->set_oversample(k);
+  return 0;
+}
+
+int grn_search_candidates(void *search, int *n, const grn_model **models) {
+  SearchHandle *h = as_search(search);
+  if (!h || !n) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  *n = (int)h->cand.models.size();
+  if (models) *models = h->cand.models.data();
+  return 0;
+}
+
+int grn_search_screen(void *search, const double *scores) {
+  SearchHandle *h = as_search(search);
+  if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
+  if (h->cand.models.empty()) return grn::set_last_error(GRN_E_INVALID, "no candidates to screen");
+  if (!scores) return grn::set_last_error(GRN_E_INVALID, "null scores");
+  for (size_t i = 0; i < h->cand.models.size(); ++i)
+    if (std::isnan(scores[i])) return grn::set_last_error(GRN_E_INVALID, "score %zu is NaN", i);
+  h->s->screen(scores);
+  h->flatten();
+  return 0;
+}
+
+int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int train_steps, int path,
+                                       grn_generation_stats *stats, grn_screen_stats *screen) {
+  SearchHandle *h = as_search(search);
+  grn::Surrogate *S = static_cast<grn::Surrogate *>(sur);
+  if (!h || !ctx || !S) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  using clock = std::chrono::steady_clock;
+  auto ms_since = [](clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+  };
+  grn_screen_stats sc{};
+  std::vector<double> predicted;  // for the children that get simulated
+  int rc;
+  if (!h->cand.models.empty()) {
+    // score every candidate, keep the best per slot
+    const auto t0 = clock::now();
+    const std::vector<grn::Individual *> &cands = h->s->candidates();
+    const int nc = (int)cands.size();
+    std::vector<float> feat((size_t)nc * MLP_FEATURES);
+    float row[MLP_WIDTH];
+    for (int i = 0; i < nc; ++i) {
+      grn::encode_model(cands[i]->model, S->n_morphogens(), row);
+      std::memcpy(feat.data() + (size_t)i * MLP_FEATURES, row, sizeof(float) * MLP_FEATURES);
+    }
+    std::vector<double> scores(nc);
+    if ((rc = S->predict(nc, feat.data(), scores.data()))) return rc;
+    // the prediction of each kept candidate, in pending order
+    const int k = h->s->oversample();
+    size_t off = 0;
+    for (const auto &isl : h->s->islands()) {
+      const int n = (int)isl.size();
+      for (int i = 0; i < n; ++i) {
+        int best = 0;
+        for (int r = 1; r < k; ++r)
+          if (scores[off + (size_t)r * n + i] < scores[off + (size_t)best * n + i]) best = r;
+        predicted.push_back(scores[off + (size_t)best * n + i]);
+      }
+      off += (size_t)k * n;
+    }
+    h->s->screen(scores.data());
+    h->flatten();
+    sc.n_candidates = nc;
+    sc.predict_ms = ms_since(t0);
+  }
+  // simulate the children
+  const auto t1 = clock::now();
+  const int n = (int)h->pend.models.size();
+  h->results.resize(n);
+  h->errors.resize(n);
+  rc = grn_evaluate_batch(ctx, n, h->pend.datasets.data(), h->pend.models.data(), h->pend.max_errors.data(),
+                          h->pend.dts.data(), path, h->results.data());
+  if (rc) return rc;
+  sc.simulate_ms = ms_since(t1);
+  sc.n_simulated = n;
+  std::vector<double> raw(n);
+  for (int i = 0; i < n; ++i) {
+    h->errors[i] = h->results[i].fitness;
+    raw[i] = h->results[i].raw_error;
+  }
+  if (!predicted.empty()) {
+    std::vector<double> pv, av;
+    for (int i = 0; i < n; ++i)
+      if (raw[i] > 0 && std::isfinite(raw[i])) {
+        pv.push_back(predicted[i]);
+        av.push_back(std::log(raw[i]));
+      }
+    sc.spearman = grn::spearman(pv, av);
+    double mae = 0;
+    for (size_t i = 0; i < pv.size(); ++i) mae += std::fabs(pv[i] - av[i]);
+    sc.mae = pv.empty() ? 0 : mae / pv.size();
+  }
+  // learn from the simulated children, then select and breed
+  const auto t2 = clock::now();
+  if ((rc = grn_surrogate_add(sur, n, h->pend.models.data(), raw.data()))) return rc;
+  double loss = 0;
+  if (train_steps > 0 && (rc = S->train(train_steps, &loss))) return rc;
+  sc.train_ms = ms_since(t2);
+  sc.loss = S->stats().last_loss;
+  sc.samples = S->stats().samples;
+  if ((rc = grn_search_advance(search, h->errors.data(), stats))) return rc;
+  if (screen) *screen = sc;
+  return 0;
 }
 
 int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links, int link_cap,

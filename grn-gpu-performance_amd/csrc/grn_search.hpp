@@ -31,6 +31,10 @@ namespace grn {
 // with the rest of the C ABI in grn_host.cpp).
 int set_last_error(int code, const char *fmt, ...);
 
+// Spearman rank correlation of two equally long samples (average ranks for
+// ties); 0 when undefined.
+double spearman(const std::vector<double> &a, const std::vector<double> &b);
+
 // xoshiro256** (Blackman & Vigna) seeded through splitmix64.
 class Rng {
  public:
@@ -62,9 +66,19 @@ class Search {
   // Models that need an error now (children or the initial population).
   int n_pending() const { return (int)pending_.size(); }
   const std::vector<Individual *> &pending() const { return pending_; }
-  // Receives the errors of pending(), in order, then selects, migrates,
-  // reproduces and refills pending().
+  // Receives the errors of pending(), in order, then selects, migrates and
+  // breeds: with oversample 1 the children go straight to pending(), with k > 1
+  // each slot gets k candidates (candidates()) until screen() picks one.
   void advance(const double *errors);
+
+  // Candidates per offspring slot for the next breeding (1 = plain search).
+  void set_oversample(int k) { oversample_ = k < 1 ? 1 : k; }
+  int oversample() const { return oversample_; }
+  // Candidates awaiting screen(): island-major, then repetition, then slot.
+  const std::vector<Individual *> &candidates() const { return cand_ptrs_; }
+  // Keeps, per slot, the candidate with the lowest score (ties: the first
+  // repetition) as the child, and fills pending().
+  void screen(const double *scores);
 
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
@@ -86,6 +100,7 @@ class Search {
   int new_label() { return ++label_counter_; }
   void prepare(Individual *ind, int island);
   void reproduce(int island);
+  void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2);
   void choose_next_generation(int island, std::vector<Individual *> *winners);
   void migrate();
   void update_pareto(const std::vector<Individual *> &fresh);
@@ -99,6 +114,9 @@ class Search {
   std::vector<Rng> rngs_;  // one stream per island, plus one for migration
   std::vector<std::vector<Individual>> islands_;
   std::vector<std::vector<Individual>> children_;
+  std::vector<std::vector<Individual>> cand_;  // oversample x island size per island
+  std::vector<Individual *> cand_ptrs_;
+  int oversample_ = 1;
   std::vector<std::vector<int>> perm_;  // shuffled parent order per island
   std::vector<Individual *> pending_;
   std::vector<Individual> pareto_;

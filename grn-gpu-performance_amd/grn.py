@@ -43,7 +43,17 @@ EXPORTED = [
     "grn_copy_probe",
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
+    "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
+    "grn_search_run_generation_screened",
+    "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
+    "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
+    "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
+    "grn_surrogate_get_params", "grn_surrogate_set_params", "grn_surrogate_get_stats",
+    "grn_mlp_gemm_bf16",
 ]
+
+MLP_FEATURES = 468
+MLP_WIDTH = 512
 
 MIGRATION_PAIRWISE, MIGRATION_RING, MIGRATION_NONE = 0, 1, 2
 
@@ -108,6 +118,24 @@ class _GenerationStats(ctypes.Structure):
                 ("reserved", ctypes.c_int)]
 
 
+class _SurrogateParams(ctypes.Structure):
+    _fields_ = [("n_morphogens", ctypes.c_int), ("batch", ctypes.c_int), ("capacity", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("lr", ctypes.c_double), ("beta1", ctypes.c_double),
+                ("beta2", ctypes.c_double), ("eps", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+class _SurrogateStats(ctypes.Structure):
+    _fields_ = [("samples", ctypes.c_long), ("train_steps", ctypes.c_long), ("last_loss", ctypes.c_double),
+                ("mu", ctypes.c_double), ("sigma", ctypes.c_double), ("train_ms", ctypes.c_double),
+                ("predict_ms", ctypes.c_double)]
+
+
+class _ScreenStats(ctypes.Structure):
+    _fields_ = [("n_candidates", ctypes.c_int), ("n_simulated", ctypes.c_int), ("spearman", ctypes.c_double),
+                ("mae", ctypes.c_double), ("loss", ctypes.c_double), ("predict_ms", ctypes.c_double),
+                ("train_ms", ctypes.c_double), ("simulate_ms", ctypes.c_double), ("samples", ctypes.c_long)]
+
+
 RESULT_DTYPE = np.dtype({"names": ["fitness", "raw_error", "change", "n_products", "steps",
                                    "experiments", "reserved"],
                          "formats": [np.float64, np.float64, np.float64, np.int32, np.int32,
@@ -157,9 +185,34 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_advance.argtypes = [vp, vp, ctypes.POINTER(_GenerationStats)]
     lib.grn_search_run_generation.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_GenerationStats)]
     lib.grn_search_best.argtypes = [vp, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.grn_search_set_oversample.argtypes = [vp, ctypes.c_int]
+    lib.grn_search_candidates.argtypes = [vp, ip, ctypes.POINTER(vp)]
+    lib.grn_search_screen.argtypes = [vp, vp]
+    lib.grn_search_run_generation_screened.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.POINTER(_GenerationStats),
+                                                       ctypes.POINTER(_ScreenStats)]
+    lib.grn_surrogate_default_params.argtypes = [ctypes.POINTER(_SurrogateParams)]
+    lib.grn_surrogate_default_params.restype = None
+    lib.grn_surrogate_create.argtypes = [ctypes.c_int, ctypes.POINTER(_SurrogateParams), ctypes.POINTER(vp)]
+    lib.grn_surrogate_destroy.argtypes = [vp]
+    lib.grn_surrogate_destroy.restype = None
+    lib.grn_surrogate_n_params.argtypes = []
+    lib.grn_surrogate_n_params.restype = ctypes.c_long
+    lib.grn_surrogate_encode.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, vp]
+    lib.grn_surrogate_add.argtypes = [vp, ctypes.c_int, vp, vp]
+    lib.grn_surrogate_train.argtypes = [vp, ctypes.c_int, dp]
+    lib.grn_surrogate_predict.argtypes = [vp, ctypes.c_int, vp, vp]
+    lib.grn_surrogate_predict_features.argtypes = [vp, ctypes.c_int, vp, vp]
+    lib.grn_surrogate_gradient.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
+    lib.grn_surrogate_get_params.argtypes = [vp, vp]
+    lib.grn_surrogate_set_params.argtypes = [vp, vp]
+    lib.grn_surrogate_get_stats.argtypes = [vp, ctypes.POINTER(_SurrogateStats)]
+    lib.grn_mlp_gemm_bf16.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp]
     for name in EXPORTED:
         if name not in ("grn_last_error", "grn_destroy", "grn_ceil_s", "grn_abi_version",
-                        "grn_search_default_params", "grn_search_destroy"):
+                        "grn_search_default_params", "grn_search_destroy", "grn_surrogate_default_params",
+                        "grn_surrogate_destroy", "grn_surrogate_n_params"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -183,6 +236,12 @@ class Model:
         self.links = np.ascontiguousarray(self.links, dtype=LINK_DTYPE)
 
     def _c(self) -> _Model:
+        # the C side reads padded records: re-lay arrays that lost the layout
+        # (np.concatenate of structured arrays can drop the padding)
+        if self.genes.dtype != GENE_DTYPE or not self.genes.flags.c_contiguous:
+            self.genes = np.ascontiguousarray(self.genes.astype(GENE_DTYPE))
+        if self.links.dtype != LINK_DTYPE or not self.links.flags.c_contiguous:
+            self.links = np.ascontiguousarray(self.links.astype(LINK_DTYPE))
         return _Model(len(self.genes), self.genes.ctypes.data if len(self.genes) else None,
                       len(self.links), self.links.ctypes.data if len(self.links) else None)
 
@@ -418,6 +477,29 @@ class Search:
         _check(self._lib.grn_search_run_generation(self._h, ctx._h, path, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}
 
+    def set_oversample(self, k: int):
+        """k candidates per offspring slot (1 = the reference's search)."""
+        _check(self._lib.grn_search_set_oversample(self._h, k))
+
+    def candidates(self) -> list:
+        n = ctypes.c_int()
+        pm = ctypes.c_void_p()
+        _check(self._lib.grn_search_candidates(self._h, ctypes.byref(n), ctypes.byref(pm)))
+        return _models_from_c(n.value, pm.value)
+
+    def screen(self, scores):
+        sc = np.ascontiguousarray(scores, dtype=np.float64)
+        _check(self._lib.grn_search_screen(self._h, sc.ctypes.data))
+
+    def run_generation_screened(self, ctx: "Context", surrogate: "Surrogate", train_steps: int = 50,
+                                path: int = PATH_AUTO):
+        """(generation stats, screening stats) of one surrogate-screened generation."""
+        st, sc = _GenerationStats(), _ScreenStats()
+        _check(self._lib.grn_search_run_generation_screened(self._h, ctx._h, surrogate._h, train_steps, path,
+                                                            ctypes.byref(st), ctypes.byref(sc)))
+        return ({k: getattr(st, k) for k, _ in _GenerationStats._fields_},
+                {k: getattr(sc, k) for k, _ in _ScreenStats._fields_})
+
     def best(self):
         """(Model, error) of the lowest-error individual (ties: lower complexity)."""
         g = np.zeros(MAX_GENES * 16, GENE_DTYPE)
@@ -426,6 +508,118 @@ class Search:
         _check(self._lib.grn_search_best(self._h, g.ctypes.data, len(g), ctypes.byref(ng), l.ctypes.data,
                                          len(l), ctypes.byref(nl), ctypes.byref(e)))
         return Model(g[:ng.value].copy(), l[:nl.value].copy()), e.value
+
+
+# ---- the learned pre-screen -----------------------------------------------------
+
+def surrogate_params(**overrides) -> dict:
+    p = _SurrogateParams()
+    load_library().grn_surrogate_default_params(ctypes.byref(p))
+    out = {k: getattr(p, k) for k, _ in _SurrogateParams._fields_}
+    for k, v in overrides.items():
+        if k not in out:
+            raise KeyError(k)
+        out[k] = v
+    return out
+
+
+def encode(model: Model, n_morphogens: int) -> np.ndarray:
+    """The surrogate's 468 features of a model."""
+    f = np.zeros(MLP_FEATURES, np.float32)
+    cm = model._c()
+    _check(load_library().grn_surrogate_encode(ctypes.byref(cm), n_morphogens, f.ctypes.data))
+    return f
+
+
+def n_surrogate_params() -> int:
+    return int(load_library().grn_surrogate_n_params())
+
+
+def mlp_gemm_bf16(a_bits: np.ndarray, bt_bits: np.ndarray, device: int = 0) -> np.ndarray:
+    """C = A . Bt^T on the surrogate's MFMA GEMM; operands as uint16 bf16 bit patterns."""
+    a = np.ascontiguousarray(a_bits, np.uint16)
+    bt = np.ascontiguousarray(bt_bits, np.uint16)
+    M, K = a.shape
+    N = bt.shape[0]
+    c = np.zeros((M, N), np.float32)
+    _check(load_library().grn_mlp_gemm_bf16(device, M, N, K, a.ctypes.data, bt.ctypes.data, c.ctypes.data))
+    return c
+
+
+def _model_array(models):
+    n = len(models)
+    return (_Model * max(n, 1))(*[m._c() for m in models])
+
+
+class Surrogate:
+    """MLP pre-screen (468 -> 4 x 512 GELU -> 1), BF16 MFMA kernels, Adam."""
+
+    def __init__(self, device: int = 0, **params):
+        lib = load_library()
+        self._lib = lib
+        self.params = surrogate_params(**params)
+        cp = _SurrogateParams(**self.params)
+        h = ctypes.c_void_p()
+        _check(lib.grn_surrogate_create(device, ctypes.byref(cp), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.grn_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, models, errors):
+        e = np.ascontiguousarray(errors, np.float64)
+        cms = _model_array(models)
+        _check(self._lib.grn_surrogate_add(self._h, len(models), ctypes.addressof(cms), e.ctypes.data))
+
+    def train(self, steps: int) -> float:
+        loss = ctypes.c_double()
+        _check(self._lib.grn_surrogate_train(self._h, steps, ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, models) -> np.ndarray:
+        out = np.zeros(max(len(models), 1), np.float64)
+        cms = _model_array(models)
+        _check(self._lib.grn_surrogate_predict(self._h, len(models), ctypes.addressof(cms), out.ctypes.data))
+        return out[:len(models)]
+
+    def predict_features(self, features: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        out = np.zeros(max(len(f), 1), np.float64)
+        _check(self._lib.grn_surrogate_predict_features(self._h, len(f), f.ctypes.data, out.ctypes.data))
+        return out[:len(f)]
+
+    def gradient(self, features: np.ndarray, targets: np.ndarray):
+        """(loss, flat gradient) of one forward/backward pass, no update."""
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, np.float64)
+        g = np.zeros(n_surrogate_params(), np.float32)
+        loss = ctypes.c_double()
+        _check(self._lib.grn_surrogate_gradient(self._h, len(f), f.ctypes.data, t.ctypes.data, g.ctypes.data,
+                                                ctypes.byref(loss)))
+        return loss.value, g
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(n_surrogate_params(), np.float32)
+        _check(self._lib.grn_surrogate_get_params(self._h, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        q = np.ascontiguousarray(p, np.float32)
+        assert q.size == n_surrogate_params()
+        _check(self._lib.grn_surrogate_set_params(self._h, q.ctypes.data))
+
+    def stats(self) -> dict:
+        s = _SurrogateStats()
+        _check(self._lib.grn_surrogate_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _SurrogateStats._fields_}
 
 
 def copy_probe(nbytes: int) -> float:

@@ -286,6 +286,86 @@ int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_
 int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
                     int link_cap, int *n_links, double *error);
 
+/* ---- learned pre-screen of offspring -------------------------------------
+ * No reference counterpart (the reference simulates every child).  An MLP
+ * (468 features, four GELU layers of 512, one output) with BF16 weights and
+ * FP32 accumulation, trained online with Adam on (model, ln error) samples
+ * that the simulator produced, ranks candidate offspring so that only the
+ * most promising are simulated.  The simulator stays the judge: every error
+ * the search keeps is a simulated one. */
+typedef struct {
+  int n_morphogens;         /* structural genes of the encoded models */
+  int batch;                /* training rows per Adam step, a multiple of 128 (4096) */
+  int capacity;             /* samples kept; the oldest are overwritten (1 << 20) */
+  int reserved;
+  double lr, beta1, beta2, eps; /* Adam: 1e-3, 0.9, 0.999, 1e-8 */
+  unsigned long long seed;  /* weight initialisation and minibatch draws */
+} grn_surrogate_params;
+
+typedef struct {
+  long samples;             /* samples in the store */
+  long train_steps;         /* Adam steps taken */
+  double last_loss;         /* mean squared error of the last step (standardised ln error) */
+  double mu, sigma;         /* standardisation: target = (ln error - mu) / sigma */
+  double train_ms;          /* device time of the last grn_surrogate_train call */
+  double predict_ms;        /* device time of the last prediction */
+} grn_surrogate_stats;
+
+void grn_surrogate_default_params(grn_surrogate_params *p);
+int grn_surrogate_create(int device, const grn_surrogate_params *p, void **surrogate);
+void grn_surrogate_destroy(void *surrogate);
+/* Length of the flat FP32 parameter vector: W_l^T (512 x 512, out x in) for
+ * l = 1..4, b_1..b_4 (512 each), output weights (512), output bias. */
+long grn_surrogate_n_params(void);
+/* The 468 features of a model (genes in use in label order, at most 12):
+ * gene s: lim, deg, 10 dif at [3s..3s+2]; link r -> t at 36 + 3 (12 r + t):
+ * type (+1 activator OR, +0.5 activator AND, -1 repressor, 0 none), K, |n|/4. */
+int grn_surrogate_encode(const grn_model *model, int n_morphogens, float *features);
+/* Adds simulated samples; errors must be finite and > 0 (others are skipped). */
+int grn_surrogate_add(void *surrogate, int n, const grn_model *models, const double *errors);
+/* Adam steps on minibatches drawn uniformly from the store. */
+int grn_surrogate_train(void *surrogate, int steps, double *last_loss);
+/* Predicted ln error of each model. */
+int grn_surrogate_predict(void *surrogate, int n, const grn_model *models, double *log_errors);
+/* The same on encoded rows (n x 468 floats). */
+int grn_surrogate_predict_features(void *surrogate, int n, const float *features, double *log_errors);
+/* One forward and backward pass on n encoded rows with standardised targets,
+ * no update: the loss (mean squared error) and its gradient. */
+int grn_surrogate_gradient(void *surrogate, int n, const float *features, const double *targets, float *grad,
+                           double *loss);
+int grn_surrogate_get_params(void *surrogate, float *params);
+int grn_surrogate_set_params(void *surrogate, const float *params);
+int grn_surrogate_get_stats(void *surrogate, grn_surrogate_stats *out);
+/* The surrogate's GEMM on its own: C[M][N] = A[M][K] . Bt[N][K]^T with BF16
+ * operands given as raw 16-bit patterns, FP32 result; M, N multiples of 128,
+ * K of 32.  For tests. */
+int grn_mlp_gemm_bf16(int device, int M, int N, int K, const unsigned short *A, const unsigned short *Bt, float *C);
+
+typedef struct {
+  int n_candidates;         /* offspring scored by the surrogate */
+  int n_simulated;          /* offspring simulated */
+  double spearman;          /* rank correlation, predicted vs simulated ln error (simulated offspring) */
+  double mae;               /* mean |predicted - simulated| ln error (simulated offspring) */
+  double loss;              /* last training loss */
+  double predict_ms, train_ms, simulate_ms; /* wall time of the phases */
+  long samples;             /* samples in the surrogate's store */
+} grn_screen_stats;
+
+/* Candidates per offspring slot (1 = the reference's search).  With k > 1
+ * every parent pair breeds k candidate pairs; grn_search_screen keeps, per
+ * slot, the candidate with the lowest score. */
+int grn_search_set_oversample(void *search, int k);
+/* Candidates awaiting grn_search_screen (valid until the next call). */
+int grn_search_candidates(void *search, int *n, const grn_model **models);
+/* Scores (lower is better) of the candidates, in order; fills pending. */
+int grn_search_screen(void *search, const double *scores);
+/* One screened generation: predict the candidates, keep the best per slot,
+ * simulate them on ctx, add them to the surrogate's samples, train
+ * train_steps Adam steps, then select, migrate and breed the next
+ * candidates.  The initial population is simulated unscreened. */
+int grn_search_run_generation_screened(void *search, void *ctx, void *surrogate, int train_steps, int path,
+                                       grn_generation_stats *stats, grn_screen_stats *screen);
+
 #ifdef __cplusplus
 }
 #endif
