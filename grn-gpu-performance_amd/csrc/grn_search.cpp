@@ -11,6 +11,7 @@
 #include <numeric>
 #include <set>
 #include <string>
+#include <thread>
 
 #include "grn_surrogate.hpp"
 
@@ -275,7 +276,7 @@ void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model 
   remove_excess(rng, c2);
 }
 
-void Search::mutate(Rng &rng, Model *m) {
+void Search::mutate(Rng &rng, Model *m, int island) {
   // Model::mutate (model.cpp:499-547)
   const int n_min = n_morph_;
   for (int i = (int)m->genes.size() - 1; i >= 0; --i) {
@@ -284,7 +285,7 @@ void Search::mutate(Rng &rng, Model *m) {
       // duplicateProduct (model.cpp:317-333)
       Gene g = m->genes[i];
       g.init = 0;
-      g.label = new_label();
+      g.label = new_label(island);
       m->genes.push_back(g);
       const int regulated = m->genes[rng.below((int)m->genes.size())].label;
       remove_link(m, g.label, regulated);
@@ -328,6 +329,8 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
   islands_.resize(p_.n_islands);
   children_.resize(p_.n_islands);
   cand_.resize(p_.n_islands);
+  label_next_.assign(p_.n_islands, 0);
+  id_next_.assign(p_.n_islands, 0);
   perm_.resize(p_.n_islands);
   for (int is = 0; is < p_.n_islands; ++is) {
     Rng &rng = rngs_[is];
@@ -350,7 +353,7 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
 
 void Search::prepare(Individual *ind, int island) {
   (void)island;
-  ind->id = next_id_++;
+  ind->id = ((uint64_t)island << 40) | ++id_next_[island];
   ind->error = -1;
   ind->complexity = ind->model.complexity_in_use(n_morph_);
   if (p_.dt_rule == 1) {
@@ -362,7 +365,8 @@ void Search::prepare(Individual *ind, int island) {
   }
 }
 
-void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2) {
+void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2,
+                        int island) {
   // one iteration of SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:70-97)
   if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
     cross(rng, a.model, b.model, &k1->model, &k2->model);
@@ -370,14 +374,14 @@ void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Indi
     k1->model = a.model;
     k2->model = b.model;
   }
-  mutate(rng, &k1->model);
-  mutate(rng, &k2->model);
+  mutate(rng, &k1->model, island);
+  mutate(rng, &k2->model, island);
   k1->parent_error = a.error;
   k2->parent_error = b.error;
   k1->dataset = a.dataset;
   k2->dataset = b.dataset;
-  prepare(k1, 0);
-  prepare(k2, 0);
+  prepare(k1, island);
+  prepare(k2, island);
 }
 
 void Search::reproduce(int is) {
@@ -391,7 +395,8 @@ void Search::reproduce(int is) {
   std::vector<Individual> &kids = children_[is];
   kids.assign(n, Individual());
   if (oversample_ <= 1) {
-    for (int i = 0; i + 1 < n; i += 2) breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1]);
+    for (int i = 0; i + 1 < n; i += 2)
+      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1], is);
     return;
   }
   // k candidate pairs per parent pair, all with the same parents
@@ -399,20 +404,45 @@ void Search::reproduce(int is) {
   cand.assign((size_t)oversample_ * n, Individual());
   for (int r = 0; r < oversample_; ++r)
     for (int i = 0; i + 1 < n; i += 2)
-      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1]);
+      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1], is);
 }
 
-void Search::screen(const double *scores) {
+static bool same_model(const Model &a, const Model &b) {
+  if (a.genes.size() != b.genes.size() || a.links.size() != b.links.size()) return false;
+  for (size_t i = 0; i < a.genes.size(); ++i) {
+    const Gene &x = a.genes[i], &y = b.genes[i];
+    if (x.label != y.label || x.init != y.init || x.lim != y.lim || x.deg != y.deg || x.dif != y.dif) return false;
+  }
+  for (size_t i = 0; i < a.links.size(); ++i) {
+    const Link &x = a.links[i], &y = b.links[i];
+    if (x.regulator != y.regulator || x.regulated != y.regulated || x.dis_const != y.dis_const ||
+        x.hill_coef != y.hill_coef || x.is_and != y.is_and)
+      return false;
+  }
+  return true;
+}
+
+void Search::screen(const double *scores, std::vector<size_t> *chosen) {
   size_t off = 0;
   pending_.clear();
+  if (chosen) chosen->clear();
   for (int is = 0; is < p_.n_islands; ++is) {
     std::vector<Individual> &cand = cand_[is];
     std::vector<Individual> &kids = children_[is];
+    const std::vector<Individual> &pop = islands_[is];
+    const std::vector<int> &perm = perm_[is];
     const int n = (int)kids.size();
     for (int i = 0; i < n; ++i) {
-      int best = 0;
-      for (int r = 1; r < oversample_; ++r)
-        if (scores[off + (size_t)r * n + i] < scores[off + (size_t)best * n + i]) best = r;
+      const Model &parent = pop[perm[i]].model;
+      int best = -1, best_any = 0;
+      for (int r = 0; r < oversample_; ++r) {
+        const double sc = scores[off + (size_t)r * n + i];
+        if (sc < scores[off + (size_t)best_any * n + i]) best_any = r;
+        if (same_model(cand[(size_t)r * n + i].model, parent)) continue;
+        if (best < 0 || sc < scores[off + (size_t)best * n + i]) best = r;
+      }
+      if (best < 0) best = best_any;
+      if (chosen) chosen->push_back(off + (size_t)best * n + i);
       kids[i] = std::move(cand[(size_t)best * n + i]);
       pending_.push_back(&kids[i]);
     }
@@ -551,8 +581,20 @@ void Search::advance(const double *errors) {
   if (p_.migration_period > 0 && generation_ % p_.migration_period == 0) migrate();
   pending_.clear();
   cand_ptrs_.clear();
+  // islands breed independently (own random stream, label stream and
+  // individuals), so they breed concurrently with identical results
+  const int n_threads = std::max(1, std::min<int>(p_.n_islands, (int)std::thread::hardware_concurrency()));
+  if (n_threads > 1) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n_threads; ++t)
+      pool.emplace_back([this, t, n_threads] {
+        for (int is = t; is < p_.n_islands; is += n_threads) reproduce(is);
+      });
+    for (std::thread &th : pool) th.join();
+  } else {
+    for (int is = 0; is < p_.n_islands; ++is) reproduce(is);
+  }
   for (int is = 0; is < p_.n_islands; ++is) {
-    reproduce(is);
     if (oversample_ <= 1)
       for (Individual &k : children_[is]) pending_.push_back(&k);
     else
@@ -764,29 +806,16 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     const auto t0 = clock::now();
     const std::vector<grn::Individual *> &cands = h->s->candidates();
     const int nc = (int)cands.size();
-    std::vector<float> feat((size_t)nc * MLP_FEATURES);
-    float row[MLP_WIDTH];
-    for (int i = 0; i < nc; ++i) {
-      grn::encode_model(cands[i]->model, S->n_morphogens(), row);
-      std::memcpy(feat.data() + (size_t)i * MLP_FEATURES, row, sizeof(float) * MLP_FEATURES);
-    }
+    std::vector<const grn::Model *> ms(nc);
+    for (int i = 0; i < nc; ++i) ms[i] = &cands[i]->model;
+    std::vector<uint16_t> rows;
+    S->encode_rows(ms, &rows);
     std::vector<double> scores(nc);
-    if ((rc = S->predict(nc, feat.data(), scores.data()))) return rc;
-    // the prediction of each kept candidate, in pending order
-    const int k = h->s->oversample();
-    size_t off = 0;
-    for (const auto &isl : h->s->islands()) {
-      const int n = (int)isl.size();
-      for (int i = 0; i < n; ++i) {
-        int best = 0;
-        for (int r = 1; r < k; ++r)
-          if (scores[off + (size_t)r * n + i] < scores[off + (size_t)best * n + i]) best = r;
-        predicted.push_back(scores[off + (size_t)best * n + i]);
-      }
-      off += (size_t)k * n;
-    }
-    h->s->screen(scores.data());
+    if ((rc = S->predict_rows(nc, rows.data(), scores.data()))) return rc;
+    std::vector<size_t> chosen;
+    h->s->screen(scores.data(), &chosen);
     h->flatten();
+    for (size_t c : chosen) predicted.push_back(scores[c]);  // in pending order
     sc.n_candidates = nc;
     sc.predict_ms = ms_since(t0);
   }
@@ -819,13 +848,26 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   }
   // learn from the simulated children, then select and breed
   const auto t2 = clock::now();
-  if ((rc = grn_surrogate_add(sur, n, h->pend.models.data(), raw.data()))) return rc;
+  {
+    std::vector<const grn::Model *> ms;
+    std::vector<double> logs;
+    for (int i = 0; i < n; ++i)
+      if (raw[i] > 0 && std::isfinite(raw[i])) {
+        ms.push_back(&h->s->pending()[i]->model);
+        logs.push_back(std::log(raw[i]));
+      }
+    std::vector<uint16_t> rows;
+    S->encode_rows(ms, &rows);
+    if ((rc = S->add_rows((int)ms.size(), rows.data(), logs.data()))) return rc;
+  }
   double loss = 0;
   if (train_steps > 0 && (rc = S->train(train_steps, &loss))) return rc;
   sc.train_ms = ms_since(t2);
   sc.loss = S->stats().last_loss;
   sc.samples = S->stats().samples;
+  const auto t3 = clock::now();
   if ((rc = grn_search_advance(search, h->errors.data(), stats))) return rc;
+  sc.breed_ms = ms_since(t3);
   if (screen) *screen = sc;
   return 0;
 }

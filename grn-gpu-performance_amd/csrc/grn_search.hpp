@@ -77,8 +77,11 @@ class Search {
   // Candidates awaiting screen(): island-major, then repetition, then slot.
   const std::vector<Individual *> &candidates() const { return cand_ptrs_; }
   // Keeps, per slot, the candidate with the lowest score (ties: the first
-  // repetition) as the child, and fills pending().
-  void screen(const double *scores);
+  // repetition) as the child, and fills pending().  Candidates identical to
+  // the slot's parent are passed over unless every candidate is one: a clone
+  // ties with its parent and changes nothing.  chosen (optional) receives
+  // the kept candidates' indices, in pending order.
+  void screen(const double *scores, std::vector<size_t> *chosen = nullptr);
 
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
@@ -89,7 +92,8 @@ class Search {
   // Operators, public for tests.
   Model create_random(Rng &rng, int n_products);
   void cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model *c2);
-  void mutate(Rng &rng, Model *m);
+  // island: whose label stream names duplicated products
+  void mutate(Rng &rng, Model *m, int island = 0);
   void remove_excess(Rng &rng, Model *m);
 
  private:
@@ -97,10 +101,13 @@ class Search {
   Link random_link(Rng &rng, int regulator, int regulated);
   void mutate_gene_params(Rng &rng, Gene *g);
   void mutate_link_params(Rng &rng, Link *l);
-  int new_label() { return ++label_counter_; }
+  // Model::createNewLabel (model.cpp:373-379) hands out 11, 12, ... from one
+  // counter; here island i draws 11 + i + n_islands * k, so labels stay
+  // unique across islands while islands breed concurrently.
+  int new_label(int island) { return 11 + island + p_.n_islands * label_next_[island]++; }
   void prepare(Individual *ind, int island);
   void reproduce(int island);
-  void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2);
+  void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2, int island);
   void choose_next_generation(int island, std::vector<Individual *> *winners);
   void migrate();
   void update_pareto(const std::vector<Individual *> &fresh);
@@ -109,8 +116,8 @@ class Search {
   grn_search_params p_;
   int n_morph_ = 0;
   int generation_ = 0;  // 0 = initial population pending
-  int label_counter_ = 10;  // Model::createNewLabel starts after 10 (model.cpp:373-379)
-  uint64_t next_id_ = 1;
+  std::vector<int> label_next_;     // per island
+  std::vector<uint64_t> id_next_;   // per island
   std::vector<Rng> rngs_;  // one stream per island, plus one for migration
   std::vector<std::vector<Individual>> islands_;
   std::vector<std::vector<Individual>> children_;

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "grn_search.hpp"
 
@@ -231,13 +232,18 @@ int Surrogate::add(int n, const float *features, const double *log_errors) {
   if (n <= 0) return 0;
   std::vector<uint16_t> rows;
   to_bf16_rows(n, n, features, &rows);
+  return add_rows(n, rows.data(), log_errors);
+}
+
+int Surrogate::add_rows(int n, const uint16_t *rows, const double *log_errors) {
+  if (n <= 0) return 0;
   std::vector<float> t(n);
   for (int i = 0; i < n; ++i) t[i] = (float)log_errors[i];
   if ((long)host_target_.size() < p_.capacity) host_target_.resize(std::min<long>(p_.capacity, n_samples_ + n));
   int done = 0;
   while (done < n) {
     const int k = (int)std::min<long>(n - done, p_.capacity - next_slot_);
-    SUR_TRY(hipMemcpyAsync(d_feat_ + (size_t)next_slot_ * MLP_WIDTH, rows.data() + (size_t)done * MLP_WIDTH,
+    SUR_TRY(hipMemcpyAsync(d_feat_ + (size_t)next_slot_ * MLP_WIDTH, rows + (size_t)done * MLP_WIDTH,
                            sizeof(uint16_t) * (size_t)k * MLP_WIDTH, hipMemcpyHostToDevice, stream_));
     SUR_TRY(hipMemcpyAsync(d_target_ + next_slot_, t.data() + done, sizeof(float) * k, hipMemcpyHostToDevice,
                            stream_));
@@ -246,9 +252,30 @@ int Surrogate::add(int n, const float *features, const double *log_errors) {
     done += k;
   }
   n_samples_ = std::min<long>(p_.capacity, n_samples_ + n);
-  SUR_TRY(hipStreamSynchronize(stream_));  // the staging vectors go out of scope
+  SUR_TRY(hipStreamSynchronize(stream_));  // the staging buffers belong to the caller
   stats_.samples = n_samples_;
   return 0;
+}
+
+void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vector<uint16_t> *rows) const {
+  const size_t n = models.size();
+  rows->assign(n * MLP_WIDTH, 0);
+  const int n_threads = (int)std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), (n + 255) / 256));
+  auto work = [&](int t) {
+    float f[MLP_WIDTH];
+    for (size_t i = t; i < n; i += n_threads) {
+      encode_model(*models[i], p_.n_morphogens, f);
+      uint16_t *r = rows->data() + i * MLP_WIDTH;
+      for (int k = 0; k < MLP_FEATURES; ++k) r[k] = float_to_bf16(f[k]);
+    }
+  };
+  if (n_threads == 1) {
+    work(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < n_threads; ++t) pool.emplace_back(work, t);
+  for (std::thread &th : pool) th.join();
 }
 
 int Surrogate::train(int steps, double *last_loss) {
@@ -301,13 +328,21 @@ int Surrogate::train(int steps, double *last_loss) {
 }
 
 int Surrogate::predict(int n, const float *features, double *log_errors) {
-  SUR_TRY(hipEventRecord(ev0_, stream_));
   std::vector<uint16_t> rows;
+  to_bf16_rows(n, n, features, &rows);
+  return predict_rows(n, rows.data(), log_errors);
+}
+
+int Surrogate::predict_rows(int n, const uint16_t *rows, double *log_errors) {
+  SUR_TRY(hipEventRecord(ev0_, stream_));
   std::vector<float> y;
   for (int b = 0; b < n; b += rows_cap_) {
     const int k = std::min(rows_cap_, n - b), kp = round_rows(k);
-    to_bf16_rows(k, kp, features + (size_t)b * MLP_FEATURES, &rows);
-    SUR_TRY(hipMemcpyAsync(d_x_, rows.data(), sizeof(uint16_t) * rows.size(), hipMemcpyHostToDevice, stream_));
+    SUR_TRY(hipMemcpyAsync(d_x_, rows + (size_t)b * MLP_WIDTH, sizeof(uint16_t) * (size_t)k * MLP_WIDTH,
+                           hipMemcpyHostToDevice, stream_));
+    if (kp > k)  // padding rows: zero features
+      SUR_TRY(hipMemsetAsync(d_x_ + (size_t)k * MLP_WIDTH, 0, sizeof(uint16_t) * (size_t)(kp - k) * MLP_WIDTH,
+                             stream_));
     int rc = forward(kp, false);
     if (rc) {
       (void)hipStreamSynchronize(stream_);

@@ -33,6 +33,11 @@ class Surrogate {
   int add(int n, const float *features, const double *log_errors);
   int train(int steps, double *last_loss);
   int predict(int n, const float *features, double *log_errors);
+  // The same on rows already encoded as BF16 (n x MLP_WIDTH).
+  int add_rows(int n, const uint16_t *rows, const double *log_errors);
+  int predict_rows(int n, const uint16_t *rows, double *log_errors);
+  // Encodes models straight into BF16 rows (n x MLP_WIDTH), on host threads.
+  void encode_rows(const std::vector<const Model *> &models, std::vector<uint16_t> *rows) const;
   // One forward and backward pass on given rows (standardised targets), no
   // update: loss and the gradient of every parameter.
   int gradient(int n, const float *features, const double *targets, float *grad, double *loss);

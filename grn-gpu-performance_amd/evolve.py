@@ -1,0 +1,135 @@
+"""Time-to-fitness: the island search run to a fitness target, plain (every
+child simulated) and surrogate-screened (k candidates per child, the best
+predicted one simulated), on the same problem and seeds.
+
+The search loop is Search::calcParalEvolution (Src/Search/search.cpp:101-216)
+with every island advanced in lockstep, one batched evaluation per
+generation; it stops when the best simulated error reaches the threshold
+(the reference stops on a perfect model, search.cpp:159-167) or after
+max_generations.  Every error the search keeps comes from the simulator.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import grn
+import workloads
+from grn import FITNESS_MSE, GENE_DTYPE, LINK_DTYPE, Model, SimParams
+
+
+def hidden_blob_gate() -> Model:
+    """Hidden 10-gene GRN with four inputs: genes 0, 1 hold an x- and a
+    y-gradient, genes 2, 3 two Gaussian blobs (no production, no decay).
+    Gene 5 is on in blob A's core, gene 6 right of x = 0.5, gene 8 above
+    y = 0.5, gene 7 in blob B's core or above y = 0.5 (an OR pair), gene 9
+    where 5 or 7 is; the target gene 4 is 9 AND NOT 6 -- the part of that
+    region left of x = 0.5 -- and diffuses slightly."""
+    genes = np.zeros(10, GENE_DTYPE)
+    genes["label"] = np.arange(10)
+    genes["lim"] = [0, 0, 0, 0, 1, 1, 1, 1, 1, 1]
+    genes["deg"] = [0, 0, 0, 0, 1, 1, 1, 1, 1, 1]
+    genes["dif"] = [0, 0, 0, 0, 0.02, 0, 0, 0, 0, 0]
+    rows = [(2, 5, 0.4, 4.0, 0), (0, 6, 0.5, 6.0, 0), (3, 7, 0.4, 4.0, 0), (1, 8, 0.5, 6.0, 0),
+            (5, 9, 0.5, 4.0, 0), (7, 9, 0.5, 4.0, 0), (9, 4, 0.4, 4.0, 1), (6, 4, 0.5, -4.0, 1),
+            (8, 7, 0.9, 1.0, 0)]
+    links = np.zeros(len(rows), LINK_DTYPE)
+    for i, r in enumerate(rows):
+        links[i] = r
+    return Model(genes, links)
+
+
+@dataclass
+class Problem:
+    name: str
+    params: SimParams
+    init: np.ndarray
+    target: np.ndarray
+    search: dict
+
+
+def scaled_config4(seed: int = 0, n_islands: int = 8, island_size: int = 256, side: int = 32,
+                   steps: int = 1000) -> Problem:
+    """The full-evolution workload at a size one GPU runs in seconds per
+    generation: four inputs (two blobs, two gradients), a hidden 10-gene GRN,
+    genes in use capped at 10."""
+    rng = np.random.default_rng(seed)
+    blobs = [workloads.gaussian_blob(side, side, rng.uniform(0.2, 0.8) * side, rng.uniform(0.2, 0.8) * side,
+                                     side / 6) for _ in range(2)]
+    init = np.stack([workloads.gradient_x(side, side), workloads.gradient_y(side, side)] + blobs)[None]
+    p = SimParams(side, side, 4, 1, steps, 1.0, fitness_kind=FITNESS_MSE)
+    search = dict(n_islands=n_islands, island_size=island_size, n_morphogens=5, n_max_products=10,
+                  n_max_links=30, init_products_min=5, init_products_max=10, migration=grn.MIGRATION_RING,
+                  migration_period=10, migration_fraction=0.05, deg_lo=0.01, dif_prob=0.3, dif_hi=0.1,
+                  k_lo=0.05, n_hi=4.0, dt_rule=1, dt_scale=0.2, dt_cap=1.0, seed=seed + 1)
+    return Problem(f"{n_islands} islands x {island_size}, {side}x{side}, {steps} steps, hidden 10-gene GRN",
+                   p, init, None, search)
+
+
+def make_target(ctx, prob: Problem) -> np.ndarray:
+    hidden = hidden_blob_gate()
+    dummy = np.zeros((1, 1, prob.params.dim_y, prob.params.dim_x))
+    ds = ctx.add_dataset(prob.params, prob.init, dummy)
+    dt = workloads.dt_rule(hidden, prob.search["dt_scale"], prob.search["dt_cap"])
+    f = ctx.simulate_fields(hidden, dataset=ds, dt=dt)
+    prob.target = f[4:5][None].copy()
+    return prob.target
+
+
+@dataclass
+class Run:
+    screened: bool
+    best: list = field(default_factory=list)      # best simulated error after each generation
+    wall: list = field(default_factory=list)      # seconds since the start, after each generation
+    cell_steps: float = 0.0
+    screen: list = field(default_factory=list)    # per-generation screening stats
+
+    def time_to(self, threshold: float):
+        for b, t in zip(self.best, self.wall):
+            if b <= threshold:
+                return t
+        return None
+
+
+def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
+           train_steps: int = 20, surrogate_batch: int = 4096, progress=None) -> Run:
+    if prob.target is None:
+        make_target(ctx, prob)
+    ds = ctx.add_dataset(prob.params, prob.init, prob.target)
+    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+    search = grn.Search(island_datasets=[ds] * prob.search["n_islands"], **prob.search)
+    sur = None
+    if screened:
+        sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], batch=surrogate_batch,
+                            seed=prob.search["seed"])
+        search.set_oversample(oversample)
+    run = Run(screened)
+    t0 = time.perf_counter()
+    for gen in range(max_generations):
+        if screened:
+            g, sc = search.run_generation_screened(ctx, sur, train_steps=train_steps)
+            run.screen.append(sc)
+        else:
+            g = search.run_generation(ctx)
+        run.cell_steps += ctx.stats()["cell_steps"]
+        run.best.append(g["min_fit"])
+        run.wall.append(time.perf_counter() - t0)
+        if progress and gen % 10 == 0:
+            progress(f"[evolve] {'screened' if screened else 'plain'} gen {gen} best {g['min_fit']:.4g} "
+                     f"t {run.wall[-1]:.1f}s")
+        if g["min_fit"] <= threshold:
+            break
+    # the best model re-simulated: the error the search reports is the simulator's
+    best, err = search.best()
+    dmax = float(np.max(best.genes["dif"])) if len(best.genes) else 0.0
+    dt = min(prob.search["dt_scale"] / dmax, prob.search["dt_cap"]) if dmax > 0 else prob.search["dt_cap"]
+    check = ctx.evaluate_batch([best], datasets=ds, dt=[dt])["fitness"][0]
+    run.verified = bool(check == err)
+    run.best_model = best
+    if sur is not None:
+        sur.close()
+    search.close()
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    return run

@@ -348,6 +348,7 @@ typedef struct {
   double mae;               /* mean |predicted - simulated| ln error (simulated offspring) */
   double loss;              /* last training loss */
   double predict_ms, train_ms, simulate_ms; /* wall time of the phases */
+  double breed_ms;          /* wall time of selection, migration and breeding */
   long samples;             /* samples in the surrogate's store */
 } grn_screen_stats;
 
