@@ -122,6 +122,65 @@ def bench_islands(args):
     ctx.close()
 
 
+def bench_time_to_fitness(args):
+    """Plain batched vs surrogate-screened evolution on the same problem and
+    seeds: wall time until the simulator-verified best error reaches each
+    threshold (the plain run's final best among them)."""
+    import grn
+    import evolve
+
+    ctx = grn.Context(0)
+    prob = evolve.scaled_config4(seed=args.seed)
+    evolve.make_target(ctx, prob)
+    runs = {}
+    for name, screened in (("plain", False), ("screened", True)):
+        log(f"[bench] {name} evolution, {args.generations} generations")
+        runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-3,
+                                   oversample=args.oversample, train_steps=args.train_steps, progress=log)
+    plain, scr = runs["plain"], runs["screened"]
+    thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
+    table = []
+    for thr in thresholds:
+        tp, ts = plain.time_to(thr), scr.time_to(thr)
+        table.append({"threshold": thr, "plain_s": tp, "screened_s": ts,
+                      "speedup": (tp / ts) if tp and ts else None})
+    sp = [s["spearman"] for s in scr.screen if s["n_candidates"]]
+    mae = [s["mae"] for s in scr.screen if s["n_candidates"]]
+    target = plain.best[-1]
+    tp, ts = plain.time_to(target), scr.time_to(target)
+    out = {
+        "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
+        "value": (tp / ts) if tp and ts else None,
+        "unit": "x (plain wall time / screened wall time to the plain run's final simulator-verified best)",
+        "n_gpus": 1,
+        "steps": args.generations,
+        "warmup": 0,
+        "ms_per_step": plain.wall[-1] / len(plain.wall) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 simulation, bf16/fp32 surrogate",
+        "data": "synthetic",
+        "config": {"workload": "time to fitness, config 4 at reduced size: " + prob.name,
+                   "oversample": args.oversample, "train_steps_per_generation": args.train_steps,
+                   "seed": args.seed},
+        "thresholds": table,
+        "plain": {"generations": len(plain.best), "wall_s": plain.wall[-1], "best": plain.best[-1],
+                  "cell_steps_per_s": plain.cell_steps / plain.wall[-1], "verified": plain.verified},
+        "screened": {"generations": len(scr.best), "wall_s": scr.wall[-1], "best": scr.best[-1],
+                     "cell_steps_per_s": scr.cell_steps / scr.wall[-1], "verified": scr.verified,
+                     "spearman_mean": float(np.mean(sp)) if sp else None,
+                     "spearman_last": sp[-1] if sp else None,
+                     "mae_mean": float(np.mean(mae)) if mae else None,
+                     "predict_ms_mean": float(np.mean([s["predict_ms"] for s in scr.screen[1:]])),
+                     "train_ms_mean": float(np.mean([s["train_ms"] for s in scr.screen[2:]])),
+                     "breed_ms_mean": float(np.mean([s["breed_ms"] for s in scr.screen[1:]]))},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,9 +190,13 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["config1", "config2"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config2", "evolve"], default="config1",
                     help="config1: one batch of 4096 models on 64x64 (the headline line); config2: "
-                         "multi-island evolution generations on mixed grid sides")
+                         "multi-island evolution generations on mixed grid sides; evolve: time to "
+                         "fitness, plain vs surrogate-screened")
+    ap.add_argument("--generations", type=int, default=400)
+    ap.add_argument("--oversample", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--islands", type=int, default=16)
     ap.add_argument("--island-size", type=int, default=1024)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -147,6 +210,8 @@ def main():
 
     if args.workload == "config2":
         return bench_islands(args)
+    if args.workload == "evolve":
+        return bench_time_to_fitness(args)
 
     import grn
     import workloads

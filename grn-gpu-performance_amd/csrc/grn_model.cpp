@@ -16,16 +16,15 @@ namespace grn {
 std::vector<int> Model::labels_in_use(int n_morphogens) const {
   // Breadth-first from the structural labels 0..nMorphogens-1 over
   // regulated -> regulator edges (model.cpp:243-270).
+  // (models hold tens of labels: a linear membership test beats a tree)
   std::vector<int> order;
-  std::set<int> seen;
-  for (int i = 0; i < n_morphogens; ++i) {
-    order.push_back(i);
-    seen.insert(i);
-  }
+  order.reserve(n_morphogens + links.size());
+  for (int i = 0; i < n_morphogens; ++i) order.push_back(i);
   for (size_t head = 0; head < order.size(); ++head) {
     const int lab = order[head];
     for (const Link &l : links)
-      if (l.regulated == lab && seen.insert(l.regulator).second) order.push_back(l.regulator);
+      if (l.regulated == lab && std::find(order.begin(), order.end(), l.regulator) == order.end())
+        order.push_back(l.regulator);
   }
   std::sort(order.begin(), order.end());
   return order;
@@ -45,13 +44,13 @@ const Gene *Model::gene_with_label(int label) const {
 }
 
 int Model::complexity_in_use(int n_morphogens) const {
-  const std::vector<int> in_use = labels_in_use(n_morphogens);
-  const std::set<int> s(in_use.begin(), in_use.end());
+  const std::vector<int> in_use = labels_in_use(n_morphogens);  // sorted
+  auto used = [&](int label) { return std::binary_search(in_use.begin(), in_use.end(), label); };
   int c = 0;
   for (const Gene &g : genes)
-    if (s.count(g.label)) c += 1 + (g.dif > 0);  // ModelProd::complexity
+    if (used(g.label)) c += 1 + (g.dif > 0);  // ModelProd::complexity
   for (const Link &l : links)
-    if (s.count(l.regulator) && s.count(l.regulated)) c += 1;  // ModelLink::complexity
+    if (used(l.regulator) && used(l.regulated)) c += 1;  // ModelLink::complexity
   return c;
 }
 

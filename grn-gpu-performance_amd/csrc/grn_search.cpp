@@ -9,7 +9,7 @@
 #include <limits>
 #include <memory>
 #include <numeric>
-#include <set>
+#include <iterator>
 #include <string>
 #include <thread>
 
@@ -85,12 +85,25 @@ int Rng::below(int n) { return (int)(((unsigned __int128)next() * (unsigned)n) >
 
 // ---- operators on models -----------------------------------------------------
 
-static bool has_label(const std::set<int> &s, int l) { return s.count(l) != 0; }
+// Label sets are small sorted vectors (a model holds tens of labels).
+typedef std::vector<int> LabelSet;
 
-static std::set<int> labels_of(const Model &m) {
-  std::set<int> s;
-  for (const Gene &g : m.genes) s.insert(g.label);
+static bool has_label(const LabelSet &s, int l) { return std::binary_search(s.begin(), s.end(), l); }
+
+static LabelSet labels_of(const Model &m) {
+  LabelSet s;
+  s.reserve(m.genes.size());
+  for (const Gene &g : m.genes) s.push_back(g.label);
+  std::sort(s.begin(), s.end());
+  s.erase(std::unique(s.begin(), s.end()), s.end());
   return s;
+}
+
+static LabelSet unite(const LabelSet &a, const LabelSet &b) {
+  LabelSet u;
+  u.reserve(a.size() + b.size());
+  std::set_union(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(u));
+  return u;
 }
 
 static int find_link(const Model &m, int regulator, int regulated) {
@@ -195,11 +208,11 @@ void Search::remove_excess(Rng &rng, Model *m) {
   }
   if (p_.n_max_links > -1) {
     // calcLinksInUse: links whose regulator and regulated are both in use
-    const std::vector<int> in_use = m->labels_in_use(n_min);
-    const std::set<int> s(in_use.begin(), in_use.end());
+    const LabelSet in_use = m->labels_in_use(n_min);  // sorted
     std::vector<int> links_in_use;
     for (size_t i = 0; i < m->links.size(); ++i)
-      if (s.count(m->links[i].regulator) && s.count(m->links[i].regulated)) links_in_use.push_back((int)i);
+      if (has_label(in_use, m->links[i].regulator) && has_label(in_use, m->links[i].regulated))
+        links_in_use.push_back((int)i);
     int to_remove = (int)links_in_use.size() - p_.n_max_links;
     std::vector<bool> drop(m->links.size(), false);
     for (int i = 0; i < to_remove; ++i) {
@@ -216,28 +229,23 @@ void Search::remove_excess(Rng &rng, Model *m) {
 
 void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model *c2) {
   // Model::cross (model.cpp:21-85): product-based uniform crossover
-  const std::set<int> prods1 = labels_of(p1), prods2 = labels_of(p2);
-  std::set<int> common, only1, only2;
-  for (int l : prods1) (has_label(prods2, l) ? common : only1).insert(l);
+  const LabelSet prods1 = labels_of(p1), prods2 = labels_of(p2);
+  LabelSet common, only1, only2;
+  for (int l : prods1) (has_label(prods2, l) ? common : only1).push_back(l);
   for (int l : prods2)
-    if (!has_label(prods1, l)) only2.insert(l);
+    if (!has_label(prods1, l)) only2.push_back(l);
   // distributeProducts: each label goes to the second set with probability 1/2
-  auto distribute = [&](const std::set<int> &from, std::set<int> *to1, std::set<int> *to2) {
-    for (int l : from) (rng.below(100) < 50 ? to2 : to1)->insert(l);
+  // (labels visited in ascending order, so every set stays sorted)
+  auto distribute = [&](const LabelSet &from, LabelSet *to1, LabelSet *to2) {
+    for (int l : from) (rng.below(100) < 50 ? to2 : to1)->push_back(l);
   };
-  std::set<int> copied, swapped, ex1to1, ex1to2, ex2to2, ex2to1;
+  LabelSet copied, swapped, ex1to1, ex1to2, ex2to2, ex2to1;
   distribute(common, &copied, &swapped);
   distribute(only1, &ex1to1, &ex1to2);
   distribute(only2, &ex2to2, &ex2to1);
-  auto unite = [](const std::set<int> &a, const std::set<int> &b) {
-    std::set<int> u = a;
-    u.insert(b.begin(), b.end());
-    return u;
-  };
   const int n_morph = n_morph_;
   // copyProductsAndLinks (model.cpp:101-184)
-  auto build = [&](const Model &from1, const std::set<int> &products1, const Model &from2,
-                   const std::set<int> &products2) {
+  auto build = [&](const Model &from1, const LabelSet &products1, const Model &from2, const LabelSet &products2) {
     Model to;
     for (const Gene &g : from1.genes)
       if (has_label(products1, g.label)) to.genes.push_back(g);
@@ -249,7 +257,7 @@ void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model 
       if (l >= n_morph) regs1.push_back(l);
     for (int l : products2)
       if (l >= n_morph) regs2.push_back(l);
-    auto copy_side = [&](const Model &from, const std::set<int> &mine, const std::set<int> &other,
+    auto copy_side = [&](const Model &from, const LabelSet &mine, const LabelSet &other,
                          const std::vector<int> &subst) {
       for (const Link &l : from.links) {
         if (!has_label(mine, l.regulated)) continue;
@@ -860,10 +868,12 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     S->encode_rows(ms, &rows);
     if ((rc = S->add_rows((int)ms.size(), rows.data(), logs.data()))) return rc;
   }
-  double loss = 0;
-  if (train_steps > 0 && (rc = S->train(train_steps, &loss))) return rc;
-  sc.train_ms = ms_since(t2);
-  sc.loss = S->stats().last_loss;
+  // the training steps run on the device while the host selects and breeds;
+  // the next prediction (or any other surrogate call) waits for them
+  if (train_steps > 0 && (rc = S->train_async(train_steps))) return rc;
+  (void)t2;
+  sc.train_ms = S->stats().train_ms;  // device time of the last completed training call
+  sc.loss = S->stats().last_loss;     // and its loss
   sc.samples = S->stats().samples;
   const auto t3 = clock::now();
   if ((rc = grn_search_advance(search, h->errors.data(), stats))) return rc;

@@ -236,6 +236,7 @@ int Surrogate::add(int n, const float *features, const double *log_errors) {
 }
 
 int Surrogate::add_rows(int n, const uint16_t *rows, const double *log_errors) {
+  if (int rc0 = finish()) return rc0;
   if (n <= 0) return 0;
   std::vector<float> t(n);
   for (int i = 0; i < n; ++i) t[i] = (float)log_errors[i];
@@ -279,7 +280,15 @@ void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vecto
 }
 
 int Surrogate::train(int steps, double *last_loss) {
-  if (steps <= 0) return 0;
+  int rc = train_async(steps);
+  if (!rc) rc = finish();
+  if (!rc && last_loss) *last_loss = stats_.last_loss;
+  return rc;
+}
+
+int Surrogate::train_async(int steps) {
+  int rc = finish();
+  if (rc || steps <= 0) return rc;
   if (n_samples_ < 1) return set_last_error(GRN_E_INVALID, "the surrogate has no samples to train on");
   // standardise the targets over the store
   double s = 0, s2 = 0;
@@ -291,18 +300,22 @@ int Surrogate::train(int steps, double *last_loss) {
   mu_ = (float)mean;
   sigma_ = (float)std::max(std::sqrt(var), 1e-3);
   const int B = p_.batch;
-  std::vector<int> idx((size_t)steps * B);
+  host_idx_.resize((size_t)steps * B);
   Rng rng(rng_);
-  for (int &v : idx) v = (int)(rng.next() % (uint64_t)n_samples_);
+  for (int &v : host_idx_) v = (int)(rng.next() % (uint64_t)n_samples_);
   rng_ = rng.next();
-  int *d_idx = nullptr;
-  int rc = dev_alloc(&d_idx, idx.size());
-  if (rc) return rc;
-  std::unique_ptr<int, void (*)(int *)> guard(d_idx, [](int *q) { (void)hipFree(q); });
-  SUR_TRY(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, stream_));
+  if (host_idx_.size() > idx_cap_) {
+    if (d_idx_) (void)hipFree(d_idx_);
+    d_idx_ = nullptr;
+    idx_cap_ = 0;
+    if ((rc = dev_alloc(&d_idx_, host_idx_.size()))) return rc;
+    idx_cap_ = host_idx_.size();
+  }
+  SUR_TRY(hipMemcpyAsync(d_idx_, host_idx_.data(), sizeof(int) * host_idx_.size(), hipMemcpyHostToDevice, stream_));
   SUR_TRY(hipEventRecord(ev0_, stream_));
+  training_ = true;  // from here on finish() must wait, also on the error paths
   for (int st = 0; st < steps; ++st) {
-    SUR_TRY(grn_mlp_gather(d_feat_, d_target_, d_idx + (size_t)st * B, B, mu_, 1.f / sigma_, d_x_, d_xt_, d_t_,
+    SUR_TRY(grn_mlp_gather(d_feat_, d_target_, d_idx_ + (size_t)st * B, B, mu_, 1.f / sigma_, d_x_, d_xt_, d_t_,
                            stream_));
     if ((rc = forward(B, true))) return rc;
     SUR_TRY(grn_mlp_out_backward(d_y_, d_t_, d_mask_, d_wout_, d_z_[MLP_HIDDEN_LAYERS - 1], 1.f / B, B, d_dy_, d_sq_,
@@ -314,16 +327,21 @@ int Surrogate::train(int steps, double *last_loss) {
   }
   SUR_TRY(grn_mlp_sum(d_sq_, B, d_loss_, stream_));
   SUR_TRY(hipEventRecord(ev1_, stream_));
+  return 0;
+}
+
+int Surrogate::finish() {
+  if (!training_) return 0;
+  training_ = false;
   float loss = 0.f, ms = 0.f;
   SUR_TRY(hipMemcpyAsync(&loss, d_loss_, sizeof(float), hipMemcpyDeviceToHost, stream_));
   SUR_TRY(hipStreamSynchronize(stream_));
   SUR_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
-  stats_.last_loss = loss / B;
+  stats_.last_loss = loss / p_.batch;
   stats_.train_steps = adam_t_;
   stats_.train_ms = ms;
   stats_.mu = mu_;
   stats_.sigma = sigma_;
-  if (last_loss) *last_loss = stats_.last_loss;
   return 0;
 }
 
@@ -334,6 +352,7 @@ int Surrogate::predict(int n, const float *features, double *log_errors) {
 }
 
 int Surrogate::predict_rows(int n, const uint16_t *rows, double *log_errors) {
+  if (int rc0 = finish()) return rc0;
   SUR_TRY(hipEventRecord(ev0_, stream_));
   std::vector<float> y;
   for (int b = 0; b < n; b += rows_cap_) {
@@ -362,6 +381,7 @@ int Surrogate::predict_rows(int n, const uint16_t *rows, double *log_errors) {
 }
 
 int Surrogate::gradient(int n, const float *features, const double *targets, float *grad, double *loss) {
+  if (int rc0 = finish()) return rc0;
   if (n < 1 || round_rows(n) > rows_cap_)
     return set_last_error(GRN_E_INVALID, "gradient takes 1 to %d rows", rows_cap_);
   const int rows = round_rows(n);
@@ -402,12 +422,14 @@ int Surrogate::gradient(int n, const float *features, const double *targets, flo
 }
 
 int Surrogate::get_params(float *p) const {
+  SUR_TRY(hipStreamSynchronize(stream_));  // pending training steps included
   SUR_TRY(hipMemcpyAsync(p, d_p_, sizeof(float) * MLP_N_PARAMS, hipMemcpyDeviceToHost, stream_));
   SUR_TRY(hipStreamSynchronize(stream_));
   return 0;
 }
 
 int Surrogate::set_params(const float *p) {
+  if (int rc0 = finish()) return rc0;
   host_p_.assign(p, p + MLP_N_PARAMS);
   return upload_bf16_copies();
 }
@@ -541,6 +563,8 @@ int grn_surrogate_set_params(void *sur, const float *params) {
 int grn_surrogate_get_stats(void *sur, grn_surrogate_stats *out) {
   grn::Surrogate *s = as_sur(sur);
   if (!s || !out) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  int rc = s->finish();
+  if (rc) return rc;
   *out = s->stats();
   return 0;
 }

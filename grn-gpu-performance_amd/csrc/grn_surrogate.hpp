@@ -32,6 +32,11 @@ class Surrogate {
 
   int add(int n, const float *features, const double *log_errors);
   int train(int steps, double *last_loss);
+  // Enqueues the training steps and returns at once; finish() (called by
+  // every other member that touches the device) waits for them and records
+  // the loss and time.
+  int train_async(int steps);
+  int finish();
   int predict(int n, const float *features, double *log_errors);
   // The same on rows already encoded as BF16 (n x MLP_WIDTH).
   int add_rows(int n, const uint16_t *rows, const double *log_errors);
@@ -74,6 +79,9 @@ class Surrogate {
   __bf16 *d_h_[MLP_HIDDEN_LAYERS] = {}, *d_ht_[MLP_HIDDEN_LAYERS] = {};
   __bf16 *d_dz_[MLP_HIDDEN_LAYERS] = {}, *d_dzt_[MLP_HIDDEN_LAYERS] = {};
   int *d_idx_ = nullptr;
+  size_t idx_cap_ = 0;
+  std::vector<int> host_idx_;  // source of the pending index upload
+  bool training_ = false;      // steps enqueued, finish() not yet called
   uint64_t rng_ = 0;
   long adam_t_ = 0;
   float mu_ = 0.f, sigma_ = 1.f;

@@ -116,7 +116,7 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         run.cell_steps += ctx.stats()["cell_steps"]
         run.best.append(g["min_fit"])
         run.wall.append(time.perf_counter() - t0)
-        if progress and gen % 10 == 0:
+        if progress and gen % 100 == 0:
             progress(f"[evolve] {'screened' if screened else 'plain'} gen {gen} best {g['min_fit']:.4g} "
                      f"t {run.wall[-1]:.1f}s")
         if g["min_fit"] <= threshold:

@@ -346,8 +346,11 @@ typedef struct {
   int n_simulated;          /* offspring simulated */
   double spearman;          /* rank correlation, predicted vs simulated ln error (simulated offspring) */
   double mae;               /* mean |predicted - simulated| ln error (simulated offspring) */
-  double loss;              /* last training loss */
-  double predict_ms, train_ms, simulate_ms; /* wall time of the phases */
+  double loss;              /* loss of the last completed training call */
+  double predict_ms;        /* wall time of encoding and scoring the candidates */
+  double train_ms;          /* device time of the last completed training call (training
+                               overlaps the host's breeding and finishes before the next prediction) */
+  double simulate_ms;       /* wall time of simulating the kept children */
   double breed_ms;          /* wall time of selection, migration and breeding */
   long samples;             /* samples in the surrogate's store */
 } grn_screen_stats;
@@ -360,10 +363,11 @@ int grn_search_set_oversample(void *search, int k);
 int grn_search_candidates(void *search, int *n, const grn_model **models);
 /* Scores (lower is better) of the candidates, in order; fills pending. */
 int grn_search_screen(void *search, const double *scores);
-/* One screened generation: predict the candidates, keep the best per slot,
- * simulate them on ctx, add them to the surrogate's samples, train
- * train_steps Adam steps, then select, migrate and breed the next
- * candidates.  The initial population is simulated unscreened. */
+/* One screened generation: predict the candidates, keep the best per slot
+ * (passing over clones of the parent), simulate them on ctx, add them to the
+ * surrogate's samples, enqueue train_steps Adam steps -- they run on the
+ * device while the host selects, migrates and breeds the next candidates --
+ * and return.  The initial population is simulated unscreened. */
 int grn_search_run_generation_screened(void *search, void *ctx, void *surrogate, int train_steps, int path,
                                        grn_generation_stats *stats, grn_screen_stats *screen);
 
