@@ -74,8 +74,24 @@ def bench_islands(args):
     w = workloads.config2(seed=args.seed, n_islands=args.islands, island_size=args.island_size)
     ctx = grn.Context(0)
     ds = workloads.add_island_datasets(ctx, w)
-    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     search = grn.Search(island_datasets=ds, **w.search)
+    tuning = []
+    if not args.no_tune:
+        # tile and workgroup shapes per size class, timed on the initial population
+        models, dsi, _, _ = search.pending()
+        t0 = time.perf_counter()
+        reports = ctx.autotune(models, datasets=dsi, sample_models=args.tune_sample, step_cap=args.tune_steps)
+        log(f"[bench] autotuned {len(reports)} size classes in {time.perf_counter() - t0:.1f} s")
+        for r in reports:
+            c, d = r["cand"][r["chosen"]], r["cand"][0]
+            kind = ["on-chip", "on-chip any width", "halo-tiled"][c["kind"]]
+            tuning.append({"grid": f"{r['dim_x']}x{r['dim_y']}", "genes": r["genes"], "models": r["n_models"],
+                           "chosen": {"kernel": kind, "cells_per_thread": c["cells_per_thread"],
+                                      "block": c["block"], "tile": [c["tile_w"], c["tile_h"]],
+                                      "interior": [c["tile_sx"], c["tile_sy"]], "halo": c["halo"]},
+                           "candidates": r["n_candidates"],
+                           "speedup_vs_default": d["ms"] / c["ms"] if d["ms"] > 0 else None})
+    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     log(f"[bench] {w.name}: warmup x{args.warmup}")
     for _ in range(args.warmup):
         search.run_generation(ctx)
@@ -114,6 +130,7 @@ def bench_islands(args):
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel_ms": float(np.mean(kernel_ms))},
         "graph_replays": replays,
+        "autotune": tuning,
         "launches_per_generation": ctx.stats()["n_launches"],
         "best_fitness": gens[-1]["min_fit"],
         "cpu_baseline": None,
@@ -197,6 +214,9 @@ def main():
     ap.add_argument("--generations", type=int, default=400)
     ap.add_argument("--oversample", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--no-tune", action="store_true", help="config2: skip the autotuner")
+    ap.add_argument("--tune-sample", type=int, default=256)
+    ap.add_argument("--tune-steps", type=int, default=100)
     ap.add_argument("--islands", type=int, default=16)
     ap.add_argument("--island-size", type=int, default=1024)
     ap.add_argument("--copy-probe-mib", type=int, default=0,

@@ -290,7 +290,8 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
     const int y = f.y0 + k;
     f.nf[k] = (float)((f.x > 0) + (f.x < W - 1) + (y > 0) + (y < H - 1));
   }
-  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt, S = D->steps;
+  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt;
+  const int S = (a.step_cap > 0 && a.step_cap < D->steps) ? a.step_cap : D->steps;
   const float mcc = D->min_conc_change;
   const double max_err = a.max_err[pi];
   const int plane_floats = G * f.plane;

@@ -91,6 +91,15 @@ struct KernelChoice {
   long slot_floats = 0;  // device-memory state per workgroup (halo-tiled)
 };
 
+// A size class of the autotuner: grid, genes in use, target genes, guard.
+struct TuneKey {
+  int W, H, G, n_tgt;
+  bool guard;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(W, H, G, n_tgt, guard) < std::tie(o.W, o.H, o.G, o.n_tgt, o.guard);
+  }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -106,6 +115,7 @@ struct Ctx {
   int tile_halo = 4;                        // Euler steps per tile load
   int launch_mode = GRN_LAUNCH_EAGER;
   std::map<std::vector<long>, hipGraphExec_t> graphs;  // by launch layout
+  std::map<TuneKey, KernelChoice> tuned;               // autotuner's choices
   grn_stats stats{};
 
   void drop_graphs() {
@@ -169,11 +179,12 @@ bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int 
   return found;
 }
 
-// Picks the kernel for a grid of W x H with G genes: an on-chip kernel when
-// the field fits (unless the tiled path is forced), else the halo-tiled one.
-int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path, KernelChoice *out) {
-  const KernelChoice none;
-  *out = none;
+// Every kernel configuration that can run a W x H grid with G genes, in the
+// order the default heuristic prefers them: on-chip kernels (exact width,
+// then any width) unless the tiled path is forced, then the halo-tiled
+// kernels with halo widths 4, 2 and 8.
+std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path) {
+  std::vector<KernelChoice> out;
   for (int pass = 0; pass < 2 && path != GRN_PATH_TILED; ++pass) {  // exact width first, then any width
     for (const GrnKernelEntry *k : c.kernels) {
       if (pass == 0 ? k->wc != W : k->wc != 0) continue;
@@ -186,25 +197,48 @@ int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int 
       const size_t stage = (size_t)n_tgt * W * H * sizeof(double);
       const size_t lds = std::max(planes, stage);
       if (lds > c.max_lds) continue;
-      out->k = k;
-      out->block = block;
-      out->lds = lds;
-      return 0;
+      KernelChoice kc;
+      kc.k = k;
+      kc.block = block;
+      kc.lds = lds;
+      out.push_back(kc);
     }
   }
   if (path != GRN_PATH_LDS) {
-    for (const GrnKernelEntry *k : c.kernels) {
-      if (k->wc != -1 || G < k->glo || G > k->ghi || k->guard != guard) continue;
-      TileGeom t;
-      if (!tile_geometry(W, H, G, k->cpt, k->ntmax, c.max_lds, c.tile_halo, &t)) continue;
-      out->k = k;
-      out->tile = t;
-      out->block = (int)(((long)t.tw * (t.th / k->cpt) + 63) / 64 * 64);
-      out->lds = (size_t)G * (t.tw + 2) * (t.th + 2) * sizeof(float);
-      out->slot_floats = 4L * G * W * H;
+    for (int halo : {c.tile_halo, 2, 8}) {
+      for (const GrnKernelEntry *k : c.kernels) {
+        if (k->wc != -1 || G < k->glo || G > k->ghi || k->guard != guard) continue;
+        TileGeom t;
+        if (!tile_geometry(W, H, G, k->cpt, k->ntmax, c.max_lds, halo, &t)) continue;
+        KernelChoice kc;
+        kc.k = k;
+        kc.tile = t;
+        kc.block = (int)(((long)t.tw * (t.th / k->cpt) + 63) / 64 * 64);
+        kc.lds = (size_t)G * (t.tw + 2) * (t.th + 2) * sizeof(float);
+        kc.slot_floats = 4L * G * W * H;
+        out.push_back(kc);
+      }
+    }
+  }
+  return out;
+}
+
+// The kernel for a grid of W x H with G genes: the autotuned choice for the
+// size class when there is one (GRN_PATH_AUTO), else the first candidate.
+int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path, KernelChoice *out) {
+  if (path == GRN_PATH_AUTO) {
+    auto it = c.tuned.find(TuneKey{W, H, G, n_tgt, guard});
+    if (it != c.tuned.end()) {
+      *out = it->second;
       return 0;
     }
   }
+  const std::vector<KernelChoice> cand = kernel_candidates(c, W, H, G, n_tgt, guard, path);
+  if (!cand.empty()) {
+    *out = cand[0];
+    return 0;
+  }
+  *out = KernelChoice();
   return fail(GRN_E_CAPACITY,
               "no on-chip kernel for a %dx%d grid with %d genes (LDS %zu bytes available)", W, H, G,
               c.max_lds);
@@ -228,9 +262,16 @@ struct Item {
 
 // Compiles, uploads, launches and collects one batch.  fields (optional)
 // receives each model's final field of experiment field_exp.
+// Options of a batch beyond the public arguments (autotuner timing runs).
+struct BatchOptions {
+  const KernelChoice *force = nullptr;  // run every model with this configuration
+  int step_cap = 0;                     // > 0: at most this many Euler steps
+};
+
 int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
               const double *max_errors, const double *dt_override, int path, grn_result *out,
-              double *fields_host, long field_stride, int field_exp, int *n_products_out) {
+              double *fields_host, long field_stride, int field_exp, int *n_products_out,
+              const BatchOptions &opt = BatchOptions()) {
   const auto t_start = std::chrono::steady_clock::now();
   if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
   std::vector<Item> items(n);
@@ -247,8 +288,12 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     rc = grn::compile_model(m, n_morph, ds, (float)dt, &items[i].prog, &items[i].info, &err);
     if (rc) return fail(rc, "model %d: %s", i, err.c_str());
     const int W = D.params.dim_x, H = D.params.dim_y;
-    rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, path, &items[i].kc);
-    if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
+    if (opt.force) {
+      items[i].kc = *opt.force;
+    } else {
+      rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, path, &items[i].kc);
+      if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
+    }
     items[i].dataset = ds;
     items[i].cost = (double)W * H * D.params.num_sim_steps * D.n_exp * items[i].info.flops_per_cell_step;
     items[i].max_err = max_errors ? max_errors[i] : std::numeric_limits<double>::infinity();
@@ -270,7 +315,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   auto class_key = [&](int a) {
     const KernelChoice &x = items[a].kc;
     const grn_sim_params &pa = c->datasets[items[a].dataset].params;
-    return std::make_tuple((const void *)x.k, x.tile.tw, x.tile.th, x.tile.sx, x.tile.sy,
+    return std::make_tuple((const void *)x.k, x.tile.tw, x.tile.th, x.tile.sx, x.tile.sy, x.tile.halo,
                            x.k->wc == -1 ? pa.dim_x : 0, x.k->wc == -1 ? pa.dim_y : 0);
   };
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
@@ -356,6 +401,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     L.fields = fields_host ? (double *)c->d_fields.p : nullptr;
     L.field_stride = field_stride;
     L.field_exp = field_exp;
+    L.step_cap = opt.step_cap;
   }
 
   hipStream_t st = c->stream;
@@ -384,7 +430,8 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     // the layout: everything a captured launch bakes in
     std::vector<long> key = {(long)c->d_batch.p, (long)c->d_out.p, (long)c->d_progs.p, (long)c->d_order.p,
                              (long)c->d_maxerr.p, (long)c->d_datasets.p, (long)c->d_scratch.p,
-                             (long)(fields_host ? c->d_fields.p : nullptr), field_stride, field_exp};
+                             (long)(fields_host ? c->d_fields.p : nullptr), field_stride, field_exp,
+                             opt.step_cap};
     for (const Launch &x : plan) {
       key.insert(key.end(), {(long)x.kc.k->fn, x.grid, x.kc.block, (long)x.kc.lds, x.kc.slot_floats,
                              x.kc.tile.tw, x.kc.tile.th, x.kc.tile.sx, x.kc.tile.sy, x.kc.tile.halo});
@@ -649,6 +696,137 @@ int grn_set_launch_mode(void *ctx, int mode) {
   if (!c) return fail(GRN_E_INVALID, "null context");
   if (mode != GRN_LAUNCH_EAGER && mode != GRN_LAUNCH_GRAPH) return fail(GRN_E_INVALID, "unknown launch mode %d", mode);
   c->launch_mode = mode;
+  return 0;
+}
+
+int grn_autotune(void *ctx, int n, const int *dataset_ids, const grn_model *models, int sample_models,
+                 int step_cap, grn_tune_class *report, int cap, int *n_classes) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || n < 0 || (n && !models) || sample_models < 1 || step_cap < 0 || cap < 0 || (cap && !report))
+    return fail(GRN_E_INVALID, "invalid autotune arguments");
+  HIP_TRY(hipSetDevice(c->device));
+  std::map<TuneKey, std::vector<int>> classes;
+  for (int i = 0; i < n; ++i) {
+    const int ds = dataset_ids ? dataset_ids[i] : 0;
+    if (ds < 0 || ds >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "model %d: unknown dataset %d", i, ds);
+    const HostDataset &D = c->datasets[ds];
+    grn::Model m;
+    int rc = to_model(&models[i], &m);
+    if (rc) return rc;
+    GrnProgram prog;
+    grn::ProgramInfo info;
+    std::string err;
+    const int n_morph = D.params.n_input_morphogens + D.params.n_target_morphogens;
+    rc = grn::compile_model(m, n_morph, ds, (float)D.params.dt, &prog, &info, &err);
+    if (rc) return fail(rc, "model %d: %s", i, err.c_str());
+    classes[TuneKey{D.params.dim_x, D.params.dim_y, prog.G, D.params.n_target_morphogens, !D.nonneg}].push_back(i);
+  }
+  const int mode = c->launch_mode;
+  c->launch_mode = GRN_LAUNCH_EAGER;  // timing runs are one-off layouts
+  int k = 0;
+  for (const auto &cls : classes) {
+    const TuneKey &key = cls.first;
+    const std::vector<int> &idx = cls.second;
+    std::vector<KernelChoice> cand = kernel_candidates(*c, key.W, key.H, key.G, key.n_tgt, key.guard, GRN_PATH_AUTO);
+    if ((int)cand.size() > GRN_TUNE_MAX_CANDIDATES) cand.resize(GRN_TUNE_MAX_CANDIDATES);
+    const int ns = std::min((int)idx.size(), sample_models);
+    std::vector<grn_model> sm(ns);
+    std::vector<int> sd(ns);
+    for (int j = 0; j < ns; ++j) {
+      sm[j] = models[idx[j]];
+      sd[j] = dataset_ids ? dataset_ids[idx[j]] : 0;
+    }
+    std::vector<grn_result> res(ns);
+    grn_tune_class rep{};
+    rep.dim_x = key.W;
+    rep.dim_y = key.H;
+    rep.genes = key.G;
+    rep.n_target_morphogens = key.n_tgt;
+    rep.guard = key.guard;
+    rep.n_models = (int)idx.size();
+    rep.n_timed = ns;
+    rep.n_candidates = (int)cand.size();
+    int best = -1;
+    for (size_t j = 0; j < cand.size(); ++j) {
+      BatchOptions o;
+      o.force = &cand[j];
+      o.step_cap = step_cap;
+      grn_tune_candidate &tc = rep.cand[j];
+      const KernelChoice &kc = cand[j];
+      tc.kind = kc.k->wc == -1 ? GRN_TUNE_TILED : (kc.k->wc == 0 ? GRN_TUNE_ONCHIP_ANY : GRN_TUNE_ONCHIP_EXACT);
+      tc.cells_per_thread = kc.k->cpt;
+      tc.block = kc.block;
+      tc.genes_lo = kc.k->glo;
+      tc.genes_hi = kc.k->ghi;
+      tc.tile_w = kc.tile.tw;
+      tc.tile_h = kc.tile.th;
+      tc.tile_sx = kc.tile.sx;
+      tc.tile_sy = kc.tile.sy;
+      tc.halo = kc.tile.halo;
+      // one untimed run (code load, buffer growth), then the timed one
+      int rc = run_batch(c, ns, sd.data(), sm.data(), nullptr, nullptr, GRN_PATH_AUTO, res.data(), nullptr, 0, 0,
+                         nullptr, o);
+      if (!rc) rc = run_batch(c, ns, sd.data(), sm.data(), nullptr, nullptr, GRN_PATH_AUTO, res.data(), nullptr, 0,
+                              0, nullptr, o);
+      if (rc) {
+        if (rc == GRN_E_DEVICE) {
+          c->launch_mode = mode;
+          return rc;
+        }
+        tc.ms = -1;
+        continue;
+      }
+      tc.ms = c->stats.kernel_ms;
+      tc.cell_steps_per_s = c->stats.cell_steps / (tc.ms * 1e-3);
+      if (best < 0 || tc.ms < rep.cand[best].ms) best = (int)j;
+    }
+    c->launch_mode = mode;
+    if (best < 0) return fail(GRN_E_CAPACITY, "no candidate ran the %dx%d class with %d genes", key.W, key.H, key.G);
+    c->launch_mode = GRN_LAUNCH_EAGER;
+    rep.chosen = best;
+    c->tuned[key] = cand[best];
+    if (k < cap) report[k] = rep;
+    ++k;
+  }
+  c->launch_mode = mode;
+  if (n_classes) *n_classes = k;
+  return 0;
+}
+
+int grn_apply_tuning(void *ctx, const grn_tune_class *classes, int n) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || n < 0 || (n && !classes)) return fail(GRN_E_INVALID, "invalid arguments");
+  std::vector<std::pair<TuneKey, KernelChoice>> picks;
+  for (int i = 0; i < n; ++i) {
+    const grn_tune_class &r = classes[i];
+    if (r.chosen < 0 || r.chosen >= r.n_candidates || r.chosen >= GRN_TUNE_MAX_CANDIDATES)
+      return fail(GRN_E_INVALID, "class %d: chosen candidate out of range", i);
+    const grn_tune_candidate &want = r.cand[r.chosen];
+    const TuneKey key{r.dim_x, r.dim_y, r.genes, r.n_target_morphogens, r.guard != 0};
+    bool found = false;
+    for (const KernelChoice &kc : kernel_candidates(*c, key.W, key.H, key.G, key.n_tgt, key.guard, GRN_PATH_AUTO)) {
+      const int kind = kc.k->wc == -1 ? GRN_TUNE_TILED : (kc.k->wc == 0 ? GRN_TUNE_ONCHIP_ANY : GRN_TUNE_ONCHIP_EXACT);
+      if (kind == want.kind && kc.k->cpt == want.cells_per_thread && kc.block == want.block &&
+          kc.k->glo == want.genes_lo && kc.k->ghi == want.genes_hi && kc.tile.tw == want.tile_w &&
+          kc.tile.th == want.tile_h && kc.tile.sx == want.tile_sx && kc.tile.sy == want.tile_sy &&
+          kc.tile.halo == want.halo) {
+        picks.push_back({key, kc});
+        found = true;
+        break;
+      }
+    }
+    if (!found)
+      return fail(GRN_E_INVALID, "class %d (%dx%d, %d genes): its chosen configuration is not available", i, r.dim_x,
+                  r.dim_y, r.genes);
+  }
+  for (auto &p : picks) c->tuned[p.first] = p.second;
+  return 0;
+}
+
+int grn_clear_tuning(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  c->tuned.clear();
   return 0;
 }
 

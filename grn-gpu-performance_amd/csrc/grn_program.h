@@ -102,7 +102,7 @@ struct GrnLaunch {
   double *fields;           // optional: [n][G][H][W] final field
   long field_stride;        // elements per model in fields
   int field_exp;            // experiment whose final field is written
-  int pad;
+  int step_cap;             // > 0: at most this many Euler steps (launch timing); 0: the dataset's
   // halo-tiled kernel only: device-memory state and tile geometry
   float *scratch;           // per workgroup: 4 planes sets (hi, lo) x (A, B)
   long slot_floats;         // floats per workgroup slot

@@ -65,7 +65,8 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   f.x = f.active ? tid % TW : 0;
   f.y0 = f.active ? (tid / TW) * CPT : 0;
   f.ubase = f.y0 * (TW + 2) + f.x;
-  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt, S = D->steps;
+  const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt;
+  const int S = (a.step_cap > 0 && a.step_cap < D->steps) ? a.step_cap : D->steps;
   const float mcc = D->min_conc_change;
   const bool early = mcc >= 0.f;
   const double max_err = a.max_err[pi];

@@ -39,7 +39,8 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
     "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
-    "grn_get_stats", "grn_set_launch_mode", "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
+    "grn_get_stats", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
+    "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
@@ -118,6 +119,41 @@ class _GenerationStats(ctypes.Structure):
                 ("reserved", ctypes.c_int)]
 
 
+TUNE_ONCHIP_EXACT, TUNE_ONCHIP_ANY, TUNE_TILED = 0, 1, 2
+TUNE_MAX_CANDIDATES = 12
+
+
+class _TuneCandidate(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("cells_per_thread", ctypes.c_int), ("block", ctypes.c_int),
+                ("genes_lo", ctypes.c_int), ("genes_hi", ctypes.c_int), ("tile_w", ctypes.c_int),
+                ("tile_h", ctypes.c_int), ("tile_sx", ctypes.c_int), ("tile_sy", ctypes.c_int),
+                ("halo", ctypes.c_int), ("ms", ctypes.c_double), ("cell_steps_per_s", ctypes.c_double)]
+
+
+class _TuneClass(ctypes.Structure):
+    _fields_ = [("dim_x", ctypes.c_int), ("dim_y", ctypes.c_int), ("genes", ctypes.c_int),
+                ("n_target_morphogens", ctypes.c_int), ("guard", ctypes.c_int), ("n_models", ctypes.c_int),
+                ("n_timed", ctypes.c_int), ("n_candidates", ctypes.c_int), ("chosen", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("cand", _TuneCandidate * TUNE_MAX_CANDIDATES)]
+
+
+def _tune_to_dict(t: _TuneClass) -> dict:
+    d = {k: getattr(t, k) for k, _ in _TuneClass._fields_ if k != "cand"}
+    d["cand"] = [{k: getattr(t.cand[j], k) for k, _ in _TuneCandidate._fields_} for j in range(t.n_candidates)]
+    return d
+
+
+def _tune_from_dict(d: dict) -> _TuneClass:
+    t = _TuneClass()
+    for k, _ in _TuneClass._fields_:
+        if k != "cand":
+            setattr(t, k, int(d[k]))
+    for j, c in enumerate(d["cand"][:TUNE_MAX_CANDIDATES]):
+        for k, _ in _TuneCandidate._fields_:
+            setattr(t.cand[j], k, c[k])
+    return t
+
+
 class _SurrogateParams(ctypes.Structure):
     _fields_ = [("n_morphogens", ctypes.c_int), ("batch", ctypes.c_int), ("capacity", ctypes.c_int),
                 ("reserved", ctypes.c_int), ("lr", ctypes.c_double), ("beta1", ctypes.c_double),
@@ -169,6 +205,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                         ctypes.c_int, ctypes.c_int, vp, ctypes.c_long, ip]
     lib.grn_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
     lib.grn_set_launch_mode.argtypes = [vp, ctypes.c_int]
+    lib.grn_autotune.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ip]
+    lib.grn_apply_tuning.argtypes = [vp, vp, ctypes.c_int]
+    lib.grn_clear_tuning.argtypes = [vp]
     lib.grn_ceil_s.argtypes = [ctypes.c_double, ctypes.c_int]
     lib.grn_ceil_s.restype = ctypes.c_double
     lib.grn_model_parse.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip]
@@ -387,6 +426,29 @@ class Context:
     def set_launch_mode(self, mode: int):
         """LAUNCH_EAGER or LAUNCH_GRAPH (capture once per kernel-class layout, then replay)."""
         _check(self._lib.grn_set_launch_mode(self._h, mode))
+
+    def autotune(self, models, datasets=None, sample_models: int = 512, step_cap: int = 200) -> list:
+        """Times every kernel configuration of every size class in the batch
+        (sample_models per class, step_cap steps) and keeps the fastest for
+        later PATH_AUTO batches.  Returns one report dict per class."""
+        n = len(models)
+        cms = (_Model * max(n, 1))(*[m._c() for m in models])
+        ds = None if datasets is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
+        cap = 256
+        rep = (_TuneClass * cap)()
+        k = ctypes.c_int()
+        _check(self._lib.grn_autotune(self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
+                                      sample_models, step_cap, ctypes.addressof(rep), cap, ctypes.byref(k)))
+        return [_tune_to_dict(rep[i]) for i in range(min(k.value, cap))]
+
+    def apply_tuning(self, reports: list):
+        """Re-applies saved autotuner reports (each class's `chosen` candidate)."""
+        arr = (_TuneClass * max(len(reports), 1))(*[_tune_from_dict(r) for r in reports])
+        _check(self._lib.grn_apply_tuning(self._h, ctypes.addressof(arr), len(reports)))
+
+    def clear_tuning(self):
+        _check(self._lib.grn_clear_tuning(self._h))
 
     def stats(self) -> dict:
         s = _Stats()

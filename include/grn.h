@@ -181,6 +181,58 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model,
  * uploaded per batch, so results are identical to GRN_LAUNCH_EAGER. */
 int grn_set_launch_mode(void *ctx, int mode);
 
+/* ---- autotuner ----------------------------------------------------------
+ * No reference counterpart (the reference launches one fixed shape).  A size
+ * class is (grid, genes in use, target genes, guard); its candidates are the
+ * kernel configurations that can run it: on-chip instantiations (cells per
+ * thread, workgroup size) and halo-tiled ones (tile window, halo width). */
+typedef enum {
+  GRN_TUNE_ONCHIP_EXACT = 0, /* on-chip kernel compiled for this grid width */
+  GRN_TUNE_ONCHIP_ANY = 1,   /* on-chip kernel for any width */
+  GRN_TUNE_TILED = 2         /* halo-tiled kernel */
+} grn_tune_kind;
+
+#define GRN_TUNE_MAX_CANDIDATES 12
+
+typedef struct {
+  int kind;                 /* grn_tune_kind */
+  int cells_per_thread;
+  int block;                /* threads per workgroup */
+  int genes_lo, genes_hi;   /* gene range of the instantiation */
+  int tile_w, tile_h;       /* halo-tiled: extended tile window (cells) */
+  int tile_sx, tile_sy;     /* halo-tiled: interior written back per tile */
+  int halo;                 /* halo-tiled: halo width = Euler steps per tile load */
+  double ms;                /* kernel time on the class sample (< 0: failed) */
+  double cell_steps_per_s;
+} grn_tune_candidate;
+
+typedef struct {
+  int dim_x, dim_y, genes, n_target_morphogens, guard;
+  int n_models;             /* models of the class in the batch */
+  int n_timed;              /* models in the timed sample */
+  int n_candidates;
+  int chosen;               /* index into cand */
+  int reserved;
+  grn_tune_candidate cand[GRN_TUNE_MAX_CANDIDATES];
+} grn_tune_class;
+
+/* Times every candidate of every size class present in the batch on at most
+ * sample_models of the class's models, with at most step_cap Euler steps
+ * (0: the datasets' own), and keeps the fastest for later GRN_PATH_AUTO
+ * evaluations on this context.  Writes up to cap class reports; *n_classes
+ * gets the number of classes. */
+int grn_autotune(void *ctx, int n_models, const int *dataset_ids, const grn_model *models, int sample_models,
+                 int step_cap, grn_tune_class *report, int cap, int *n_classes);
+
+/* Applies saved reports (e.g. from an earlier process): for each class the
+ * candidate cand[chosen] is looked up among the configurations this build
+ * can run for the class and kept as its choice; GRN_E_INVALID if one is not
+ * available.  Nothing is timed. */
+int grn_apply_tuning(void *ctx, const grn_tune_class *classes, int n);
+
+/* Forgets the autotuner's choices (the default heuristic applies again). */
+int grn_clear_tuning(void *ctx);
+
 /* Timing and work counters of the last evaluate call on this context. */
 int grn_get_stats(void *ctx, grn_stats *out);
 

@@ -125,7 +125,8 @@ def test_struct_layouts_match_header(tmp_path):
     mirrors = {"grn_sim_params": grn._SimParams, "grn_model": grn._Model, "grn_result": grn._Result,
                "grn_stats": grn._Stats, "grn_search_params": grn._SearchParams,
                "grn_generation_stats": grn._GenerationStats, "grn_surrogate_params": grn._SurrogateParams,
-               "grn_surrogate_stats": grn._SurrogateStats, "grn_screen_stats": grn._ScreenStats}
+               "grn_surrogate_stats": grn._SurrogateStats, "grn_screen_stats": grn._ScreenStats,
+               "grn_tune_candidate": grn._TuneCandidate, "grn_tune_class": grn._TuneClass}
     dtypes = {"grn_gene": grn.GENE_DTYPE, "grn_link": grn.LINK_DTYPE}
     structs = {k: [f for f, _ in v._fields_] for k, v in mirrors.items()}
     structs.update({k: list(v.names) for k, v in dtypes.items()})
