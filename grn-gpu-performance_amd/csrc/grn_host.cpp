@@ -138,7 +138,7 @@ struct Ctx {
 void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
   const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w48,
                                               grn_field_table_w64, grn_field_table_wany,
-                                              grn_field_table_tiled};
+                                              grn_field_table_tiled512, grn_field_table_tiled};
   for (auto fn : tables) {
     int n = 0;
     const GrnKernelEntry *t = fn(&n);
@@ -182,7 +182,8 @@ bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int 
 // Every kernel configuration that can run a W x H grid with G genes, in the
 // order the default heuristic prefers them: on-chip kernels (exact width,
 // then any width) unless the tiled path is forced, then the halo-tiled
-// kernels with halo widths 4, 2 and 8.
+// kernels with halo widths 4, 2 and 8 (512-thread instantiations first:
+// they keep their state in registers without spilling).
 std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path) {
   std::vector<KernelChoice> out;
   for (int pass = 0; pass < 2 && path != GRN_PATH_TILED; ++pass) {  // exact width first, then any width

@@ -22,6 +22,7 @@ extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
 
 // Zeroes out[0 .. batch->n_total) on stream (grn_probe.hip).
 hipError_t grn_launch_clear_results(const GrnBatch *batch, GrnModelOut *out, hipStream_t stream);
