@@ -692,6 +692,36 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, doubl
   return 0;
 }
 
+int grn_test_performance(void *ctx, int dataset_id, const char *model_text, int n, double expected,
+                         double *max_dif, double *ms) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !model_text || n < 1) return fail(GRN_E_INVALID, "invalid arguments");
+  grn::Model m;
+  std::string err;
+  if (!grn::Model::parse(model_text, &m, &err)) return fail(GRN_E_MODEL, "%s", err.c_str());
+  std::vector<grn_gene> genes;
+  std::vector<grn_link> links;
+  for (const grn::Gene &g : m.genes) genes.push_back(grn_gene{g.label, g.init, g.lim, g.deg, g.dif});
+  for (const grn::Link &l : m.links)
+    links.push_back(grn_link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and ? 1 : 0});
+  const grn_model gm{(int)genes.size(), genes.data(), (int)links.size(), links.data()};
+  std::vector<grn_model> models(n, gm);
+  std::vector<int> ds(n, dataset_id);
+  std::vector<double> me(n, 1000.0);  // Individual without parents (individual.cpp:23-24)
+  std::vector<grn_result> res(n);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = run_batch(c, n, ds.data(), models.data(), me.data(), nullptr, GRN_PATH_AUTO, res.data(), nullptr, 0, 0,
+                     nullptr);
+  if (rc) return rc;
+  double worst = 0;
+  for (const grn_result &r : res) worst = std::max(worst, std::fabs(r.fitness - expected));
+  if (max_dif) *max_dif = worst;
+  if (ms) *ms = c->stats.kernel_ms;
+  if (worst != 0)
+    return fail(GRN_E_WRONG, "WRONG ERROR: largest difference %.17g from the expected %.17g", worst, expected);
+  return 0;
+}
+
 int grn_set_launch_mode(void *ctx, int mode) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(GRN_E_INVALID, "null context");

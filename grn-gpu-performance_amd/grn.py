@@ -39,7 +39,7 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
     "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
-    "grn_get_stats", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
+    "grn_get_stats", "grn_test_performance", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
     "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
@@ -205,6 +205,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
                                         ctypes.c_int, ctypes.c_int, vp, ctypes.c_long, ip]
     lib.grn_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
     lib.grn_set_launch_mode.argtypes = [vp, ctypes.c_int]
+    lib.grn_test_performance.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, dp, dp]
     lib.grn_autotune.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ip]
     lib.grn_apply_tuning.argtypes = [vp, vp, ctypes.c_int]
     lib.grn_clear_tuning.argtypes = [vp]
@@ -422,6 +423,16 @@ class Context:
         _check(self._lib.grn_simulate_fields(self._h, dataset, ctypes.byref(cm), dt, experiment, path,
                                              buf.ctypes.data, cap, ctypes.byref(g)))
         return buf[: g.value * p.dim_x * p.dim_y].reshape(g.value, p.dim_y, p.dim_x).copy()
+
+    def test_performance(self, model_text: str, n: int, expected: float, dataset: int = 0):
+        """ErrorCalculatorMultiThread::testPerformance: n copies in one batch,
+        exact comparison.  Returns (ok, largest |fitness - expected|, kernel ms)."""
+        dif, ms = ctypes.c_double(), ctypes.c_double()
+        rc = self._lib.grn_test_performance(self._h, dataset, model_text.encode(), n, expected,
+                                            ctypes.byref(dif), ctypes.byref(ms))
+        if rc not in (0, -11):
+            _check(rc)
+        return rc == 0, dif.value, ms.value
 
     def set_launch_mode(self, mode: int):
         """LAUNCH_EAGER or LAUNCH_GRAPH (capture once per kernel-class layout, then replay)."""

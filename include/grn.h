@@ -51,7 +51,8 @@ typedef enum {
   GRN_E_CAPACITY = -3, /* model or grid beyond the kernels' limits */
   GRN_E_DEVICE = -4,   /* HIP runtime error */
   GRN_E_NOMEM = -5,    /* host or device allocation failed */
-  GRN_E_RESULT = -6    /* the kernel did not write a result */
+  GRN_E_RESULT = -6,   /* the kernel did not write a result */
+  GRN_E_WRONG = -11    /* a known-answer check failed (testPerformance's -11) */
 } grn_status;
 
 typedef enum {
@@ -172,6 +173,15 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
 int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model,
                         double dt_override, int experiment, int path,
                         double *fields, long capacity, int *n_products);
+
+/* ErrorCalculatorMultiThread::testPerformance
+ * (Src/UI/Evolution/errorcalculatormultithread.cpp:59-90): evaluates n
+ * copies of the model in `model_text` as one batch (maxError 1000, a
+ * parentless individual), each fitness compared for exact equality with
+ * `expected`.  *max_dif gets the largest |fitness - expected|, *ms the
+ * kernel time; returns GRN_E_WRONG when any copy differs. */
+int grn_test_performance(void *ctx, int dataset_id, const char *model_text, int n, double expected,
+                         double *max_dif, double *ms);
 
 /* No reference counterpart: how grn_evaluate_batch (and the search calls
  * that use it) issue a batch's launches.  GRN_LAUNCH_GRAPH captures the

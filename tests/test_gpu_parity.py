@@ -264,3 +264,22 @@ def test_tiled_early_stop_and_guard(ctx):
     m = grn.Model(g, np.zeros(0, grn.LINK_DTYPE))
     ds2, got2, want2 = run_both(ctx, q, init2, tgt2, [m])
     check_fitness(got2["raw_error"], want2["raw_error"])
+
+
+def test_known_answer_self_test(ctx):
+    """testPerformance: many copies of the KAT model in one batch all give
+    exactly the single evaluation's fitness, which matches the oracle."""
+    kat = open(__file__.replace("test_gpu_parity.py", "golden/kat_model.txt")).read().strip()
+    m = grn.Model.from_text(kat)
+    W = H = 24
+    p = grn.SimParams(W, H, 1, 1, 300, 0.05, fitness_kind=grn.FITNESS_REF, error_precision=10)
+    init = workloads.gradient_x(W, H)[None, None]
+    target = workloads.gaussian_blob(W, H, 8.0, 12.0, 4.0)[None, None]
+    ds = ctx.add_dataset(p, init, target)
+    one = ctx.evaluate(m, 1000.0, dataset=ds)["fitness"]
+    ok, dif, ms = ctx.test_performance(kat, 300, one, dataset=ds)
+    assert ok and dif == 0.0 and ms > 0
+    ok, dif, _ = ctx.test_performance(kat, 8, one * (1 + 1e-9), dataset=ds)
+    assert not ok and dif > 0
+    want = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, max_error=1000.0)[0]
+    assert abs(one - want["fitness"]) <= 1e-5 * abs(want["fitness"])
