@@ -31,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 # x 1.0 (scale factors from the copy-kernel calibration) as measured by
 # rocprofv3 --pmc; the commands and raw counts are in
 # profiles/field_kernel_config1_64x64.md.
-MEASURED_TRAFFIC_BYTES = 667.8e6
+MEASURED_TRAFFIC_BYTES = 1156.3e6
 
 
 def log(msg):
@@ -198,6 +198,58 @@ def bench_time_to_fitness(args):
     ctx.close()
 
 
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def bench_surrogate(args):
+    """The surrogate's BF16-MFMA kernels on their own: Adam training steps
+    (batch 4096: gather, 4 forward GEMMs, 7 backward GEMMs, reductions, Adam)
+    and inference over 131072 encoded candidates (8 x 16384 offspring)."""
+    import grn
+    import workloads
+
+    rng = np.random.default_rng(args.seed)
+    sur = grn.Surrogate(0, n_morphogens=4, batch=4096, capacity=1 << 16, seed=args.seed)
+    models = workloads._population(rng, 4096, (4, 12), (1, 40), 4)
+    sur.add(models, rng.uniform(1e-3, 1.0, len(models)))
+    log("[bench] surrogate warmup")
+    sur.train(max(args.warmup, 1) * 5)
+    n_steps = args.steps * 20
+    t0 = time.perf_counter()
+    sur.train(n_steps)
+    train_wall = time.perf_counter() - t0
+    st = sur.stats()
+    mat = grn.MLP_WIDTH * grn.MLP_WIDTH
+    # one step: 4 forward GEMMs + 4 weight-gradient GEMMs + 3 input-gradient GEMMs
+    step_flops = 11 * 2.0 * 4096 * mat
+    feats = rng.uniform(-1, 1, (131072, grn.MLP_FEATURES)).astype(np.float32)
+    sur.predict_features(feats[:8192])
+    t0 = time.perf_counter()
+    sur.predict_features(feats)
+    pred_wall = time.perf_counter() - t0
+    pred_ms = sur.stats()["predict_ms"]
+    pred_flops = 4 * 2.0 * 131072 * mat
+    train_tflops = step_flops * n_steps / (st["train_ms"] * 1e-3) / 1e12
+    out = {
+        "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
+        "value": 131072 / (pred_ms * 1e-3),
+        "unit": "candidates scored/s (device time, 131072 candidates)",
+        "n_gpus": 1, "steps": n_steps, "warmup": 5, "ms_per_step": st["train_ms"] / n_steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "surrogate MLP 468 -> 4 x 512 GELU -> 1, Adam batch 4096; inference 131072 rows"},
+        "train": {"ms_per_step": st["train_ms"] / n_steps, "wall_ms_per_step": train_wall / n_steps * 1e3,
+                  "samples_per_s": 4096 * n_steps / (st["train_ms"] * 1e-3), "tflops": train_tflops},
+        "predict": {"device_ms": pred_ms, "wall_ms": pred_wall * 1e3,
+                    "tflops": pred_flops / (pred_ms * 1e-3) / 1e12},
+        "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": BF16_DENSE_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": train_tflops / BF16_DENSE_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    sur.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,10 +259,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["config1", "config2", "evolve"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config2", "evolve", "surrogate"], default="config1",
                     help="config1: one batch of 4096 models on 64x64 (the headline line); config2: "
                          "multi-island evolution generations on mixed grid sides; evolve: time to "
-                         "fitness, plain vs surrogate-screened")
+                         "fitness, plain vs surrogate-screened; surrogate: the MLP's training and "
+                         "inference kernels")
     ap.add_argument("--generations", type=int, default=400)
     ap.add_argument("--oversample", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=10)
@@ -232,6 +285,8 @@ def main():
         return bench_islands(args)
     if args.workload == "evolve":
         return bench_time_to_fitness(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
 
     import grn
     import workloads
