@@ -94,13 +94,16 @@ struct Model {
 
   __device__ __forceinline__ int pc() const { return WC ? WC + 2 : W + 2; }
 
+  // Only diffusing genes are read back from the LDS mirror.
   __device__ __forceinline__ void store_planes() {
     if (!active) return;
+    const int dm = P->dif_mask;
     float *pl = sh.lds + ubase + pc() + 1;
 #pragma unroll
     for (int p = 0; p < G; ++p)
+      if (dm & (1 << p))
 #pragma unroll
-      for (int k = 0; k < CPT; ++k) pl[p * plane + k * pc()] = hi[p][k];
+        for (int k = 0; k < CPT; ++k) pl[p * plane + k * pc()] = hi[p][k];
   }
 
   // One Euler step of every owned cell, processed cell by cell down the
@@ -186,7 +189,7 @@ struct Model {
           const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
           const float sv = c + yv;
           const float nl = yv - (sv - c);
-          const bool clamp = (sv < min_conc) && (clamp_any_r || R != 0.f);
+          const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));  // branch-free
           hi[p][k] = clamp ? 0.f : sv;
           lo[p][k] = clamp ? 0.f : nl;
           if (TRACK) cmax = __builtin_fmaxf(cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);

@@ -116,6 +116,7 @@ struct Ctx {
   int launch_mode = GRN_LAUNCH_EAGER;
   std::map<std::vector<long>, hipGraphExec_t> graphs;  // by launch layout
   std::map<TuneKey, KernelChoice> tuned;               // autotuner's choices
+  mutable std::map<std::pair<TuneKey, int>, KernelChoice> defaults;  // default choice per class and path
   grn_stats stats{};
 
   void drop_graphs() {
@@ -234,9 +235,16 @@ int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int 
       return 0;
     }
   }
+  const std::pair<TuneKey, int> dk{TuneKey{W, H, G, n_tgt, guard}, path};
+  auto hit = c.defaults.find(dk);
+  if (hit != c.defaults.end()) {
+    *out = hit->second;
+    return 0;
+  }
   const std::vector<KernelChoice> cand = kernel_candidates(c, W, H, G, n_tgt, guard, path);
   if (!cand.empty()) {
     *out = cand[0];
+    c.defaults[dk] = cand[0];
     return 0;
   }
   *out = KernelChoice();

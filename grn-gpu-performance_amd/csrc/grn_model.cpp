@@ -256,6 +256,12 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
     }
   }
   out->n_links = nl;
+  for (int i = 0; i < G; ++i) {
+    const GrnGene &gp = out->gene[i];
+    if (gp.n_or + gp.n_and > 0) out->hill_mask |= 1 << i;
+    if (gp.dif != 0.f) out->dif_mask |= 1 << i;
+  }
+  for (int l = 0; l < nl; ++l) out->reg_mask |= 1 << out->link[l].r;
   inf.L = nl;
   inf.flops_per_cell_step = 5.0 * inf.L + (inf.L - inf.G_h) + 8.0 * inf.G + 7.0 * inf.D;
   if (info) *info = inf;

@@ -47,7 +47,10 @@ struct GrnProgram {
   int n_links;        // links stored below
   int dataset;        // experiment set
   float dt;           // Euler time step
-  int pad[4];
+  int reg_mask;       // bit g: gene g regulates a stored link (its log2 is needed)
+  int hill_mask;      // bit g: gene g has an activator (else h = 0: no Hill terms)
+  int dif_mask;       // bit g: gene g diffuses (its LDS plane is read)
+  int pad;
   GrnGene gene[GRN_PMAX_GENES];
   GrnLink link[GRN_PMAX_LINKS];
 };
