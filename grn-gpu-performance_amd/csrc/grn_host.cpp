@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -287,15 +288,38 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   for (int i = 0; i < n; ++i) {
     const int ds = dataset_ids ? dataset_ids[i] : 0;
     if (ds < 0 || ds >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "model %d: unknown dataset %d", i, ds);
+    items[i].dataset = ds;
+  }
+  // Compile every model (independent work: host threads for large batches).
+  std::vector<int> crc(n, 0);
+  std::vector<std::string> cerr(n);
+  auto compile_range = [&](int t, int nt) {
+    for (int i = t; i < n; i += nt) {
+      const HostDataset &D = c->datasets[items[i].dataset];
+      grn::Model m;
+      if (!grn::model_from_abi(&models[i], &m)) {
+        crc[i] = GRN_E_INVALID;
+        cerr[i] = "invalid grn_model";
+        continue;
+      }
+      const double dt = (dt_override && dt_override[i] > 0) ? dt_override[i] : D.params.dt;
+      const int n_morph = D.params.n_input_morphogens + D.params.n_target_morphogens;
+      crc[i] = grn::compile_model(m, n_morph, items[i].dataset, (float)dt, &items[i].prog, &items[i].info, &cerr[i]);
+    }
+  };
+  const int n_threads = n >= 1024 ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
+  if (n_threads > 1) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n_threads; ++t) pool.emplace_back(compile_range, t, n_threads);
+    for (std::thread &th : pool) th.join();
+  } else {
+    compile_range(0, 1);
+  }
+  for (int i = 0; i < n; ++i) {
+    if (crc[i]) return fail(crc[i], "model %d: %s", i, cerr[i].c_str());
+    const int ds = items[i].dataset;
     const HostDataset &D = c->datasets[ds];
-    grn::Model m;
-    int rc = to_model(&models[i], &m);
-    if (rc) return rc;
-    const double dt = (dt_override && dt_override[i] > 0) ? dt_override[i] : D.params.dt;
-    std::string err;
-    const int n_morph = D.params.n_input_morphogens + D.params.n_target_morphogens;
-    rc = grn::compile_model(m, n_morph, ds, (float)dt, &items[i].prog, &items[i].info, &err);
-    if (rc) return fail(rc, "model %d: %s", i, err.c_str());
+    int rc;
     const int W = D.params.dim_x, H = D.params.dim_y;
     if (opt.force) {
       items[i].kc = *opt.force;
@@ -303,7 +327,6 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
       rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, path, &items[i].kc);
       if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
     }
-    items[i].dataset = ds;
     items[i].cost = (double)W * H * D.params.num_sim_steps * D.n_exp * items[i].info.flops_per_cell_step;
     items[i].max_err = max_errors ? max_errors[i] : std::numeric_limits<double>::infinity();
   }

@@ -343,6 +343,19 @@ class SimParams:
                           self.dist_error_threshold, self.error_precision, self.fitness_kind)
 
 
+class ModelBatch:
+    """Models marshalled once into the C array grn_evaluate_batch reads (the
+    way a C++ caller keeps its grn_model array); the models must not change
+    while the batch is in use."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        self.array = (_Model * max(len(self.models), 1))(*[m._c() for m in self.models])
+
+    def __len__(self):
+        return len(self.models)
+
+
 class Context:
     """One device context: stream, uploaded experiment sets, device buffers."""
 
@@ -396,9 +409,10 @@ class Context:
 
     def evaluate_batch(self, models, datasets=None, max_errors=None, dt=None,
                        path: int = PATH_AUTO) -> np.ndarray:
-        """Batched path: all models in persistent launches, one sync."""
+        """Batched path: all models in persistent launches, one sync.
+        `models` is a list of Model or a ModelBatch (marshalled once)."""
         n = len(models)
-        cms = (_Model * max(n, 1))(*[m._c() for m in models])
+        cms = models.array if isinstance(models, ModelBatch) else (_Model * max(n, 1))(*[m._c() for m in models])
         ds = None if datasets is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
         me = None if max_errors is None else np.ascontiguousarray(

@@ -283,3 +283,15 @@ def test_known_answer_self_test(ctx):
     assert not ok and dif > 0
     want = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, max_error=1000.0)[0]
     assert abs(one - want["fitness"]) <= 1e-5 * abs(want["fitness"])
+
+
+def test_model_batch_equals_list(ctx):
+    """A ModelBatch (marshalled once) evaluates exactly like the model list,
+    also when reused."""
+    w = workloads.config0(seed=4, n_models=48)
+    workloads.make_target(ctx, w)
+    ds = ctx.add_dataset(w.params, w.init, w.target)
+    a = ctx.evaluate_batch(w.models, datasets=ds)
+    b = grn.ModelBatch(w.models)
+    assert np.array_equal(ctx.evaluate_batch(b, datasets=ds), a)
+    assert np.array_equal(ctx.evaluate_batch(b, datasets=ds), a)
