@@ -38,7 +38,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, budget_s=15.0):
+def cpu_baseline(w, budget_s=25.0):
     """FP64 oracle on the host cores over a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import grn_oracle as ora
@@ -268,6 +268,7 @@ def main():
     ap.add_argument("--oversample", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="config2: skip the autotuner")
+    ap.add_argument("--tune", action="store_true", help="config1: autotune the size classes first")
     ap.add_argument("--tune-sample", type=int, default=256)
     ap.add_argument("--tune-steps", type=int, default=100)
     ap.add_argument("--islands", type=int, default=16)
@@ -299,6 +300,14 @@ def main():
     flops_per_model = np.array([m.info(4)[2] for m in w.models])
 
     batch = grn.ModelBatch(w.models)  # the population, marshalled once as a C caller holds it
+    tuning = None
+    if args.tune:
+        reports = ctx.autotune(batch, datasets=ds, sample_models=args.tune_sample, step_cap=args.tune_steps)
+        tuning = [{"genes": r["genes"], "models": r["n_models"],
+                   "chosen": {k: r["cand"][r["chosen"]][k] for k in ("kind", "cells_per_thread", "block")},
+                   "speedup_vs_default": (r["cand"][0]["ms"] / r["cand"][r["chosen"]]["ms"])
+                   if r["cand"][0]["ms"] > 0 else None} for r in reports]
+        log(f"[bench] autotuned {len(reports)} size classes")
     log(f"[bench] warmup x{args.warmup}")
     for _ in range(args.warmup):
         ctx.evaluate_batch(batch, datasets=ds)
@@ -361,6 +370,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC_BYTES if args.models == 4096 else None,
                      "kernel_ms": float(np.mean(kernel_ms)),
                      "flops_per_cell_step_mean": float(np.mean(flops_per_model))},
+        "autotune": tuning,
         "plain": {"value": plain_rate, "unit": "cell-steps/s", "models": len(sample),
                   "pattern": "one model per launch, synchronise per model"},
         "batched_over_plain": value / plain_rate if plain_rate else None,

@@ -5,6 +5,7 @@
 
 static const GrnKernelEntry kTable[] = {
     GRN_TILED_ENTRY(4, 512, 2, 8, false),
+    GRN_TILED_ENTRY(4, 512, 9, 10, false),  // 11+ genes would spill at 4 cells per thread
     GRN_TILED_ENTRY(2, 512, 9, 16, false),
 };
 

@@ -457,7 +457,7 @@ class Context:
         (sample_models per class, step_cap steps) and keeps the fastest for
         later PATH_AUTO batches.  Returns one report dict per class."""
         n = len(models)
-        cms = (_Model * max(n, 1))(*[m._c() for m in models])
+        cms = models.array if isinstance(models, ModelBatch) else (_Model * max(n, 1))(*[m._c() for m in models])
         ds = None if datasets is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
         cap = 256
