@@ -124,7 +124,9 @@ struct Model {
     return best;
   }
 
-  template <bool TRACK>
+  // FAST: the program guarantees a nonnegative state (no minConc clamp) and
+  // no zero Hill exponent (no log2 floor), see GrnProgram::fast.
+  template <bool TRACK, bool FAST>
   __device__ __forceinline__ float step() {
     typedef typename LgVec<G>::T lg_t;
     constexpr int CB = cell_block();
@@ -141,7 +143,19 @@ struct Model {
 #pragma unroll
       for (int j = 0; j < CB; ++j)
 #pragma unroll
-        for (int g = 0; g < G; ++g) lg[j][g] = __builtin_fmaxf(hw_log2(hi[g][k0 + j]), -kMaxF);
+        for (int g = 0; g < G; ++g) lg[j][g] = FAST ? hw_log2(hi[g][k0 + j]) : __builtin_fmaxf(hw_log2(hi[g][k0 + j]), -kMaxF);
+      // On-chip kernels: define the spare lanes and pin each vector as one
+      // register tuple, so the indexed reads below use it in place instead of
+      // the compiler re-assembling a copy before every gene's link loop.  (The
+      // tiled kernels keep more state live and would spill instead.)
+      if constexpr (!TILED) {
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+#pragma unroll
+          for (int g = G; g < lg_width(); ++g) lg[j][g] = 0.f;
+          asm volatile("" : "+v"(lg[j]));
+        }
+      }
 
 #pragma unroll
       for (int p = 0; p < G; ++p) {
@@ -189,9 +203,14 @@ struct Model {
           const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
           const float sv = c + yv;
           const float nl = yv - (sv - c);
-          const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));  // branch-free
-          hi[p][k] = clamp ? 0.f : sv;
-          lo[p][k] = clamp ? 0.f : nl;
+          if (FAST) {
+            hi[p][k] = sv;
+            lo[p][k] = nl;
+          } else {
+            const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));  // branch-free
+            hi[p][k] = clamp ? 0.f : sv;
+            lo[p][k] = clamp ? 0.f : nl;
+          }
           if (TRACK) cmax = __builtin_fmaxf(cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
         }
         // Take this gene's results here so the compiler cannot sink its
@@ -222,18 +241,19 @@ struct Model {
   // Simulator::simulate (simulator.cpp:54-63): run while step < S and the
   // last step's max change exceeds minConcChange.  A negative threshold can
   // never stop the loop, so then only the final step needs the reduction.
+  template <bool FAST>
   __device__ __forceinline__ float run_steps(int S, float mcc, int *steps_done) {
     float chg = 1.0f;
     int s = 0;
     if (mcc >= 0.f) {
       while (s < S && chg > mcc) {
-        chg = step<true>();
+        chg = step<true, FAST>();
         ++s;
       }
     } else {
-      for (; s < S - 1; ++s) step<false>();
+      for (; s < S - 1; ++s) step<false, FAST>();
       if (S > 0) {
-        chg = step<true>();
+        chg = step<true, FAST>();
         ++s;
       }
     }
@@ -323,7 +343,7 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
     __syncthreads();
 
     int sdone = 0;
-    const float chg = f.run_steps(S, mcc, &sdone);
+    const float chg = P->fast ? f.template run_steps<true>(S, mcc, &sdone) : f.template run_steps<false>(S, mcc, &sdone);
     steps_total += sdone;
     chg_acc += (double)chg;
 

@@ -319,6 +319,8 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     if (crc[i]) return fail(crc[i], "model %d: %s", i, cerr[i].c_str());
     const int ds = items[i].dataset;
     const HostDataset &D = c->datasets[ds];
+    // the fast step also needs a nonnegative start and no positive minConc
+    if (!(D.nonneg && D.params.min_conc <= 0)) items[i].prog.fast = 0;
     int rc;
     const int W = D.params.dim_x, H = D.params.dim_y;
     if (opt.force) {

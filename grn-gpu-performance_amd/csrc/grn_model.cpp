@@ -262,6 +262,22 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
     if (gp.dif != 0.f) out->dif_mask |= 1 << i;
   }
   for (int l = 0; l < nl; ++l) out->reg_mask |= 1 << out->link[l].r;
+  // The fast step drops the minConc clamp and the log2 floor.  Exact update
+  // c' = c (1 - dt deg - dt dif n) + dt (lim h + dif sum(neighbours)) stays
+  // >= 0 from a nonnegative state when lim, deg, dif >= 0 and
+  // dt (deg + 4 dif) <= 0.99 (the margin covers the rounding of dt*R); the
+  // caller adds the dataset's conditions (nonnegative inputs, minConc <= 0).
+  // log2(0) = -inf is harmless once no link has a zero Hill exponent
+  // (a*(-inf) + b = +-inf, and h = 0 or 1 as x^n/(x^n+K^n) gives).
+  bool fast = true;
+  for (int i = 0; i < G; ++i) {
+    const GrnGene &gp = out->gene[i];
+    if (!(gp.lim >= 0.f && gp.deg >= 0.f && gp.dif >= 0.f)) fast = false;
+    if (!((double)dt * ((double)gp.deg + 4.0 * (double)gp.dif) <= 0.99)) fast = false;
+  }
+  for (int l = 0; l < nl; ++l)
+    if (out->link[l].a == 0.f) fast = false;
+  out->fast = fast ? 1 : 0;
   inf.L = nl;
   inf.flops_per_cell_step = 5.0 * inf.L + (inf.L - inf.G_h) + 8.0 * inf.G + 7.0 * inf.D;
   if (info) *info = inf;

@@ -50,7 +50,8 @@ struct GrnProgram {
   int reg_mask;       // bit g: gene g regulates a stored link (its log2 is needed)
   int hill_mask;      // bit g: gene g has an activator (else h = 0: no Hill terms)
   int dif_mask;       // bit g: gene g diffuses (its LDS plane is read)
-  int pad;
+  int fast;           // 1: the state provably stays >= 0 and no Hill exponent is 0,
+                      // so the step needs neither the minConc clamp nor the log2 floor
   GrnGene gene[GRN_PMAX_GENES];
   GrnLink link[GRN_PMAX_LINKS];
 };

@@ -121,12 +121,14 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
           __syncthreads();
           f.store_planes();
           __syncthreads();
+          // (the general step only: a second step variant in these larger
+          // kernels costs register spills that outweigh what it saves)
           float c = 0.f;
           for (int t = 0; t < Tb; ++t) {
             if (track && t == Tb - 1)
-              c = f.template step<true>();
+              c = f.template step<true, false>();
             else
-              f.template step<false>();
+              f.template step<false, false>();
           }
           cblk = __builtin_fmaxf(cblk, c);
 #pragma unroll
