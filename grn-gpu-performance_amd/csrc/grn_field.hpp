@@ -160,32 +160,56 @@ struct Model {
 #pragma unroll
       for (int p = 0; p < G; ++p) {
         const GrnGene gp = Pq->gene[p];
-        float tor[CB], tand[CB];
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-          tor[j] = gp.tor0;
-          tand[j] = gp.tand0;
-        }
-        // one 16-byte scalar load per link: (r, a, b, pad)
+        // h for the block's cells: the reference's fold min(max_OR H, min_AND H)
+        // in exponent form, T = max(min_OR t, max_AND t), h = 1/(1 + 2^T).
+        // Each group starts from its first link (no +-inf seeds); a gene
+        // without an activator has no stored links and h = 0, without the
+        // exponential and reciprocal.  One 16-byte scalar load per link:
+        // (r, a, b, pad).
         const float4 *lk = reinterpret_cast<const float4 *>(Pq->link);
-        for (int l = gp.link_begin; l < gp.link_begin + gp.n_or; ++l) {
-          const float4 L = lk[l];
-          const int r = __float_as_int(L.x);
+        const int lb = gp.link_begin, nor = gp.n_or, nand = gp.n_and;
+        float hv[CB];
+        if (nor + nand > 0) {
+          float T[CB];
+          if (nor > 0) {
+            const float4 L0 = lk[lb];
+            const int r0 = __float_as_int(L0.x);
 #pragma unroll
-          for (int j = 0; j < CB; ++j) tor[j] = __builtin_fminf(tor[j], __builtin_fmaf(L.y, lg[j][r], L.z));
-        }
-        for (int l = gp.link_begin + gp.n_or; l < gp.link_begin + gp.n_or + gp.n_and; ++l) {
-          const float4 L = lk[l];
-          const int r = __float_as_int(L.x);
+            for (int j = 0; j < CB; ++j) T[j] = __builtin_fmaf(L0.y, lg[j][r0], L0.z);
+            for (int l = lb + 1; l < lb + nor; ++l) {
+              const float4 L = lk[l];
+              const int r = __float_as_int(L.x);
 #pragma unroll
-          for (int j = 0; j < CB; ++j) tand[j] = __builtin_fmaxf(tand[j], __builtin_fmaf(L.y, lg[j][r], L.z));
+              for (int j = 0; j < CB; ++j) T[j] = __builtin_fminf(T[j], __builtin_fmaf(L.y, lg[j][r], L.z));
+            }
+          }
+          if (nand > 0) {
+            float U[CB];
+            const float4 L0 = lk[lb + nor];
+            const int r0 = __float_as_int(L0.x);
+#pragma unroll
+            for (int j = 0; j < CB; ++j) U[j] = __builtin_fmaf(L0.y, lg[j][r0], L0.z);
+            for (int l = lb + nor + 1; l < lb + nor + nand; ++l) {
+              const float4 L = lk[l];
+              const int r = __float_as_int(L.x);
+#pragma unroll
+              for (int j = 0; j < CB; ++j) U[j] = __builtin_fmaxf(U[j], __builtin_fmaf(L.y, lg[j][r], L.z));
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) T[j] = nor > 0 ? __builtin_fmaxf(T[j], U[j]) : U[j];
+          }
+#pragma unroll
+          for (int j = 0; j < CB; ++j) hv[j] = hw_rcp(1.0f + hw_exp2(T[j]));
+        } else {
+#pragma unroll
+          for (int j = 0; j < CB; ++j) hv[j] = 0.f;
         }
         const float *pl = sh.lds + p * plane + ubase;
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
           const int k = k0 + j;
           const float c = hi[p][k];
-          const float h = hw_rcp(1.0f + hw_exp2(__builtin_fmaxf(tor[j], tand[j])));
+          const float h = hv[j];
           float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
           const float left = pl[(k + 1) * pc()];
           const float right = pl[(k + 1) * pc() + 2];
