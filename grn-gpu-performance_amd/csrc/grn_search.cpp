@@ -189,8 +189,9 @@ Model Search::create_random(Rng &rng, int n_products) {
 void Search::remove_excess(Rng &rng, Model *m) {
   // Model::removeExcess(nMinProducts, nMaxProducts, nMaxLinks) (model.cpp:550-574)
   const int n_min = n_morph_;
+  if (p_.n_max_products < 0 && p_.n_max_links < 0) return;
+  LabelSet in_use = m->labels_in_use(n_min);  // sorted; recomputed only after removals
   if (p_.n_max_products > -1) {
-    std::vector<int> in_use = m->labels_in_use(n_min);
     int to_remove = (int)in_use.size() - p_.n_max_products;
     if (to_remove > 0) {
       in_use.erase(std::remove_if(in_use.begin(), in_use.end(), [&](int l) { return l < n_min; }), in_use.end());
@@ -204,16 +205,17 @@ void Search::remove_excess(Rng &rng, Model *m) {
           }
         in_use.erase(in_use.begin() + k);
       }
+      in_use = m->labels_in_use(n_min);
     }
   }
   if (p_.n_max_links > -1) {
     // calcLinksInUse: links whose regulator and regulated are both in use
-    const LabelSet in_use = m->labels_in_use(n_min);  // sorted
     std::vector<int> links_in_use;
     for (size_t i = 0; i < m->links.size(); ++i)
       if (has_label(in_use, m->links[i].regulator) && has_label(in_use, m->links[i].regulated))
         links_in_use.push_back((int)i);
     int to_remove = (int)links_in_use.size() - p_.n_max_links;
+    if (to_remove <= 0) return;
     std::vector<bool> drop(m->links.size(), false);
     for (int i = 0; i < to_remove; ++i) {
       const int k = rng.below((int)links_in_use.size());
@@ -247,6 +249,8 @@ void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model 
   // copyProductsAndLinks (model.cpp:101-184)
   auto build = [&](const Model &from1, const LabelSet &products1, const Model &from2, const LabelSet &products2) {
     Model to;
+    to.genes.reserve(products1.size() + products2.size() + 2);
+    to.links.reserve(from1.links.size() + from2.links.size() + 2);
     for (const Gene &g : from1.genes)
       if (has_label(products1, g.label)) to.genes.push_back(g);
     for (const Gene &g : from2.genes)
@@ -352,6 +356,7 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
       ind.model = create_random(rng, lo + rng.below(hi - lo + 1));
       ind.dataset = island_datasets ? island_datasets[is] : 0;
       prepare(&ind, is);
+      ind.complexity = ind.model.complexity_in_use(n_morph_);
       islands_[is].push_back(std::move(ind));
     }
   }
@@ -360,10 +365,11 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
 }
 
 void Search::prepare(Individual *ind, int island) {
-  (void)island;
   ind->id = ((uint64_t)island << 40) | ++id_next_[island];
   ind->error = -1;
-  ind->complexity = ind->model.complexity_in_use(n_morph_);
+  // the in-use complexity is computed once an individual is pending
+  // (a screened-out candidate never needs it)
+  ind->complexity = -1;
   if (p_.dt_rule == 1) {
     double dmax = 0;
     for (const Gene &g : ind->model.genes) dmax = std::max(dmax, g.dif);
@@ -373,14 +379,22 @@ void Search::prepare(Individual *ind, int island) {
   }
 }
 
+// A copy with room for the genes and links a mutation may add.
+static void copy_model(const Model &from, Model *to) {
+  to->genes.reserve(from.genes.size() + 2);
+  to->links.reserve(from.links.size() + 4);
+  to->genes.assign(from.genes.begin(), from.genes.end());
+  to->links.assign(from.links.begin(), from.links.end());
+}
+
 void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2,
                         int island) {
   // one iteration of SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:70-97)
   if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
     cross(rng, a.model, b.model, &k1->model, &k2->model);
   } else {
-    k1->model = a.model;
-    k2->model = b.model;
+    copy_model(a.model, &k1->model);
+    copy_model(b.model, &k2->model);
   }
   mutate(rng, &k1->model, island);
   mutate(rng, &k2->model, island);
@@ -405,6 +419,7 @@ void Search::reproduce(int is) {
   if (oversample_ <= 1) {
     for (int i = 0; i + 1 < n; i += 2)
       breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1], is);
+    for (Individual &k : kids) k.complexity = k.model.complexity_in_use(n_morph_);
     return;
   }
   // k candidate pairs per parent pair, all with the same parents
@@ -452,6 +467,7 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
       if (best < 0) best = best_any;
       if (chosen) chosen->push_back(off + (size_t)best * n + i);
       kids[i] = std::move(cand[(size_t)best * n + i]);
+      kids[i].complexity = kids[i].model.complexity_in_use(n_morph_);
       pending_.push_back(&kids[i]);
     }
     off += cand.size();
