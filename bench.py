@@ -31,7 +31,7 @@ FP32_PEAK_TFLOPS = 157.3
 # x 1.0 (scale factors from the copy-kernel calibration) as measured by
 # rocprofv3 --pmc; the commands and raw counts are in
 # profiles/field_kernel_config1_64x64.md.
-MEASURED_TRAFFIC_BYTES = 1156.3e6
+MEASURED_TRAFFIC_BYTES = 1398.0e6
 
 
 def log(msg):
@@ -153,7 +153,8 @@ def bench_time_to_fitness(args):
     for name, screened in (("plain", False), ("screened", True)):
         log(f"[bench] {name} evolution, {args.generations} generations")
         runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-3,
-                                   oversample=args.oversample, train_steps=args.train_steps, progress=log)
+                                   oversample=args.oversample, train_steps=args.train_steps,
+                                   screen_fraction=args.screen_fraction, progress=log)
     plain, scr = runs["plain"], runs["screened"]
     thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
     table = []
@@ -179,7 +180,8 @@ def bench_time_to_fitness(args):
         "dtype": "fp32 simulation, bf16/fp32 surrogate",
         "data": "synthetic",
         "config": {"workload": "time to fitness, config 4 at reduced size: " + prob.name,
-                   "oversample": args.oversample, "train_steps_per_generation": args.train_steps,
+                   "oversample": args.oversample, "screen_fraction": args.screen_fraction,
+                   "train_steps_per_generation": args.train_steps,
                    "seed": args.seed},
         "thresholds": table,
         "plain": {"generations": len(plain.best), "wall_s": plain.wall[-1], "best": plain.best[-1],
@@ -266,6 +268,8 @@ def main():
                          "inference kernels")
     ap.add_argument("--generations", type=int, default=400)
     ap.add_argument("--oversample", type=int, default=8)
+    ap.add_argument("--screen-fraction", type=float, default=1.0,
+                    help="evolve: fraction of each island's slots simulated per generation")
     ap.add_argument("--train-steps", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true", help="config2: skip the autotuner")
     ap.add_argument("--tune", action="store_true", help="config1: autotune the size classes first")

@@ -406,7 +406,7 @@ void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Indi
   prepare(k2, island);
 }
 
-void Search::reproduce(int is) {
+void Search::reproduce(int is, uint16_t *rows) {
   // SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:63-101): shuffle
   // the parents, breed consecutive pairs; child i belongs to parent perm[i]
   Rng &rng = rngs_[is];
@@ -428,6 +428,8 @@ void Search::reproduce(int is) {
   for (int r = 0; r < oversample_; ++r)
     for (int i = 0; i + 1 < n; i += 2)
       breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1], is);
+  if (rows)
+    for (size_t c = 0; c < cand.size(); ++c) encoder_(cand[c].model, n_morph_, rows + c * (size_t)row_width_);
 }
 
 static bool same_model(const Model &a, const Model &b) {
@@ -449,31 +451,59 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
   size_t off = 0;
   pending_.clear();
   if (chosen) chosen->clear();
+  std::vector<int> pick;
+  std::vector<double> gain;
+  std::vector<int> order;
   for (int is = 0; is < p_.n_islands; ++is) {
     std::vector<Individual> &cand = cand_[is];
     std::vector<Individual> &kids = children_[is];
     const std::vector<Individual> &pop = islands_[is];
     const std::vector<int> &perm = perm_[is];
     const int n = (int)kids.size();
+    pick.assign(n, 0);
+    gain.assign(n, 0.0);
     for (int i = 0; i < n; ++i) {
-      const Model &parent = pop[perm[i]].model;
+      const Individual &parent = pop[perm[i]];
       int best = -1, best_any = 0;
       for (int r = 0; r < oversample_; ++r) {
         const double sc = scores[off + (size_t)r * n + i];
         if (sc < scores[off + (size_t)best_any * n + i]) best_any = r;
-        if (same_model(cand[(size_t)r * n + i].model, parent)) continue;
+        if (same_model(cand[(size_t)r * n + i].model, parent.model)) continue;
         if (best < 0 || sc < scores[off + (size_t)best * n + i]) best = r;
       }
-      if (best < 0) best = best_any;
-      if (chosen) chosen->push_back(off + (size_t)best * n + i);
-      kids[i] = std::move(cand[(size_t)best * n + i]);
-      kids[i].complexity = kids[i].model.complexity_in_use(n_morph_);
-      pending_.push_back(&kids[i]);
+      pick[i] = best < 0 ? best_any : best;
+      const double g = scores[off + (size_t)pick[i] * n + i] - parent.log_raw;
+      gain[i] = std::isnan(g) ? std::numeric_limits<double>::infinity() : g;
+    }
+    // the slots to simulate, in slot order
+    int keep = n;
+    if (screen_fraction_ < 1.0) keep = std::max(1, std::min(n, (int)std::ceil(screen_fraction_ * n)));
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0);
+    if (keep < n) {
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gain[a] < gain[b]; });
+      order.resize(keep);
+      std::sort(order.begin(), order.end());
+    }
+    size_t next = 0;
+    for (int i = 0; i < n; ++i) {
+      if (next < order.size() && order[next] == i) {
+        ++next;
+        const size_t c = off + (size_t)pick[i] * n + i;
+        if (chosen) chosen->push_back(c);
+        kids[i] = std::move(cand[c - off]);
+        kids[i].complexity = kids[i].model.complexity_in_use(n_morph_);
+        pending_.push_back(&kids[i]);
+      } else {
+        // not simulated: cannot replace its parent
+        kids[i].error = std::numeric_limits<double>::infinity();
+      }
     }
     off += cand.size();
     cand.clear();
   }
   cand_ptrs_.clear();
+  rows_written_ = false;
 }
 
 void Search::choose_next_generation(int is, std::vector<Individual *> *winners) {
@@ -605,18 +635,34 @@ void Search::advance(const double *errors) {
   if (p_.migration_period > 0 && generation_ % p_.migration_period == 0) migrate();
   pending_.clear();
   cand_ptrs_.clear();
+  // candidate feature rows: island is's block starts at row_off[is]
+  std::vector<uint16_t *> rows(p_.n_islands, nullptr);
+  rows_written_ = false;
+  if (rows_ && encoder_ && oversample_ > 1) {
+    size_t total = 0;
+    for (int is = 0; is < p_.n_islands; ++is) total += (size_t)oversample_ * islands_[is].size();
+    if (total <= rows_cap_) {
+      size_t at = 0;
+      for (int is = 0; is < p_.n_islands; ++is) {
+        rows[is] = rows_ + at * (size_t)row_width_;
+        at += (size_t)oversample_ * islands_[is].size();
+      }
+      rows_written_ = true;
+    }
+  }
+  rows_ = nullptr;  // the buffer is lent for this call only
   // islands breed independently (own random stream, label stream and
   // individuals), so they breed concurrently with identical results
   const int n_threads = std::max(1, std::min<int>(p_.n_islands, (int)std::thread::hardware_concurrency()));
   if (n_threads > 1) {
     std::vector<std::thread> pool;
     for (int t = 0; t < n_threads; ++t)
-      pool.emplace_back([this, t, n_threads] {
-        for (int is = t; is < p_.n_islands; is += n_threads) reproduce(is);
+      pool.emplace_back([this, t, n_threads, &rows] {
+        for (int is = t; is < p_.n_islands; is += n_threads) reproduce(is, rows[is]);
       });
     for (std::thread &th : pool) th.join();
   } else {
-    for (int is = 0; is < p_.n_islands; ++is) reproduce(is);
+    for (int is = 0; is < p_.n_islands; ++is) reproduce(is, rows[is]);
   }
   for (int is = 0; is < p_.n_islands; ++is) {
     if (oversample_ <= 1)
@@ -677,9 +723,19 @@ struct SearchHandle {
   std::vector<grn_result> results;
   std::vector<double> errors;
 
+  bool cand_built = false;
+  const void *rows_lender = nullptr;  // the surrogate whose staging rows the last breeding wrote
+
+  // pending() is flattened at once; candidates() only when asked for
   void flatten() {
     pend.build(s->pending());
-    cand.build(s->candidates());
+    cand = FlatModels();
+    cand_built = false;
+  }
+  const FlatModels &candidates() {
+    if (!cand_built) cand.build(s->candidates());
+    cand_built = true;
+    return cand;
   }
 };
 
@@ -759,7 +815,7 @@ int grn_search_pending(void *search, int *n, const grn_model **models, const int
 int grn_search_advance(void *search, const double *errors, grn_generation_stats *stats) {
   SearchHandle *h = as_search(search);
   if (!h || (!errors && !h->pend.models.empty())) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  if (h->pend.models.empty() && !h->cand.models.empty())
+  if (h->pend.models.empty() && !h->s->candidates().empty())
     return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
   for (size_t i = 0; i < h->pend.models.size(); ++i)
     if (!(errors[i] >= 0) || std::isinf(errors[i]))
@@ -773,7 +829,7 @@ int grn_search_advance(void *search, const double *errors, grn_generation_stats 
 int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats) {
   SearchHandle *h = as_search(search);
   if (!h || !ctx) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  if (h->pend.models.empty() && !h->cand.models.empty())
+  if (h->pend.models.empty() && !h->s->candidates().empty())
     return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
   const int n = (int)h->pend.models.size();
   h->results.resize(n);
@@ -796,20 +852,40 @@ int grn_search_set_oversample(void *search, int k) {
 int grn_search_candidates(void *search, int *n, const grn_model **models) {
   SearchHandle *h = as_search(search);
   if (!h || !n) return grn::set_last_error(GRN_E_INVALID, "null argument");
-  *n = (int)h->cand.models.size();
-  if (models) *models = h->cand.models.data();
+  const FlatModels &c = h->candidates();
+  *n = (int)c.models.size();
+  if (models) *models = c.models.data();
   return 0;
 }
 
 int grn_search_screen(void *search, const double *scores) {
   SearchHandle *h = as_search(search);
   if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
-  if (h->cand.models.empty()) return grn::set_last_error(GRN_E_INVALID, "no candidates to screen");
+  const size_t nc = h->s->candidates().size();
+  if (!nc) return grn::set_last_error(GRN_E_INVALID, "no candidates to screen");
   if (!scores) return grn::set_last_error(GRN_E_INVALID, "null scores");
-  for (size_t i = 0; i < h->cand.models.size(); ++i)
+  for (size_t i = 0; i < nc; ++i)
     if (std::isnan(scores[i])) return grn::set_last_error(GRN_E_INVALID, "score %zu is NaN", i);
   h->s->screen(scores);
   h->flatten();
+  return 0;
+}
+
+int grn_search_set_screen_fraction(void *search, double fraction) {
+  SearchHandle *h = as_search(search);
+  if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
+  if (!(fraction > 0 && fraction <= 1)) return grn::set_last_error(GRN_E_INVALID, "fraction must lie in (0, 1]");
+  h->s->set_screen_fraction(fraction);
+  return 0;
+}
+
+int grn_search_record_raw(void *search, const double *raw_errors) {
+  SearchHandle *h = as_search(search);
+  if (!h || (!raw_errors && h->s->n_pending())) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  const std::vector<grn::Individual *> &pend = h->s->pending();
+  for (size_t i = 0; i < pend.size(); ++i)
+    pend[i]->log_raw = raw_errors[i] > 0 && std::isfinite(raw_errors[i]) ? std::log(raw_errors[i])
+                                                                           : std::numeric_limits<double>::quiet_NaN();
   return 0;
 }
 
@@ -824,19 +900,26 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   };
   grn_screen_stats sc{};
   std::vector<double> predicted;  // for the children that get simulated
+  std::vector<size_t> chosen;     // their candidate indices
+  const uint16_t *cand_rows = nullptr;
   int rc;
-  if (!h->cand.models.empty()) {
+  const int nc = (int)h->s->candidates().size();
+  if (nc) {
     // score every candidate, keep the best per slot
     const auto t0 = clock::now();
-    const std::vector<grn::Individual *> &cands = h->s->candidates();
-    const int nc = (int)cands.size();
-    std::vector<const grn::Model *> ms(nc);
-    for (int i = 0; i < nc; ++i) ms[i] = &cands[i]->model;
-    std::vector<uint16_t> rows;
-    S->encode_rows(ms, &rows);
+    if (h->s->candidate_rows_written() && h->rows_lender == S) {
+      cand_rows = S->staging_rows(0);  // written while breeding
+    } else {
+      uint16_t *buf = S->staging_rows(nc);
+      if (!buf) return GRN_E_NOMEM;
+      const std::vector<grn::Individual *> &cands = h->s->candidates();
+      std::vector<const grn::Model *> ms(nc);
+      for (int i = 0; i < nc; ++i) ms[i] = &cands[i]->model;
+      S->encode_rows(ms, buf);
+      cand_rows = buf;
+    }
     std::vector<double> scores(nc);
-    if ((rc = S->predict_rows(nc, rows.data(), scores.data()))) return rc;
-    std::vector<size_t> chosen;
+    if ((rc = S->predict_rows(nc, cand_rows, scores.data()))) return rc;
     h->s->screen(scores.data(), &chosen);
     h->flatten();
     for (size_t c : chosen) predicted.push_back(scores[c]);  // in pending order
@@ -858,6 +941,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     h->errors[i] = h->results[i].fitness;
     raw[i] = h->results[i].raw_error;
   }
+  if ((rc = grn_search_record_raw(search, raw.data()))) return rc;
   if (!predicted.empty()) {
     std::vector<double> pv, av;
     for (int i = 0; i < n; ++i)
@@ -870,28 +954,46 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     for (size_t i = 0; i < pv.size(); ++i) mae += std::fabs(pv[i] - av[i]);
     sc.mae = pv.empty() ? 0 : mae / pv.size();
   }
-  // learn from the simulated children, then select and breed
-  const auto t2 = clock::now();
+  // learn from the simulated children (their rows are candidate rows
+  // already encoded, except for the initial population), then select and breed
   {
     std::vector<const grn::Model *> ms;
+    std::vector<int> idx;
     std::vector<double> logs;
     for (int i = 0; i < n; ++i)
       if (raw[i] > 0 && std::isfinite(raw[i])) {
-        ms.push_back(&h->s->pending()[i]->model);
+        idx.push_back(i);
         logs.push_back(std::log(raw[i]));
       }
-    std::vector<uint16_t> rows;
-    S->encode_rows(ms, &rows);
-    if ((rc = S->add_rows((int)ms.size(), rows.data(), logs.data()))) return rc;
+    std::vector<uint16_t> rows(idx.size() * (size_t)MLP_WIDTH);
+    if (cand_rows) {
+      for (size_t j = 0; j < idx.size(); ++j)
+        std::memcpy(rows.data() + j * MLP_WIDTH, cand_rows + chosen[idx[j]] * MLP_WIDTH,
+                    sizeof(uint16_t) * MLP_WIDTH);
+    } else {
+      for (int i : idx) ms.push_back(&h->s->pending()[i]->model);
+      S->encode_rows(ms, rows.data());
+    }
+    if ((rc = S->add_rows((int)idx.size(), rows.data(), logs.data()))) return rc;
   }
   // the training steps run on the device while the host selects and breeds;
   // the next prediction (or any other surrogate call) waits for them
   if (train_steps > 0 && (rc = S->train_async(train_steps))) return rc;
-  (void)t2;
   sc.train_ms = S->stats().train_ms;  // device time of the last completed training call
   sc.loss = S->stats().last_loss;     // and its loss
   sc.samples = S->stats().samples;
   const auto t3 = clock::now();
+  // the next candidates are encoded into the surrogate's pinned staging rows
+  // by the breeding threads
+  if (h->s->oversample() > 1) {
+    size_t total = 0;
+    for (const auto &isl : h->s->islands()) total += isl.size();
+    total *= (size_t)h->s->oversample();
+    if (uint16_t *buf = S->staging_rows(total)) {
+      h->s->set_candidate_rows(buf, total, MLP_WIDTH, &grn::encode_model_bf16);
+      h->rows_lender = S;
+    }
+  }
   if ((rc = grn_search_advance(search, h->errors.data(), stats))) return rc;
   sc.breed_ms = ms_since(t3);
   if (screen) *screen = sc;

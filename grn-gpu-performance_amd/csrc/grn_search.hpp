@@ -19,6 +19,7 @@
 #pragma once
 
 #include <cstdint>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -57,7 +58,13 @@ struct Individual {
   int dataset = 0;
   double dt = 0;               // per-model time step (0: the dataset's)
   uint64_t id = 0;
+  // ln of the simulated raw error where the caller recorded it (NaN if not):
+  // the scale a surrogate predicts on, for comparing a child with its parent
+  double log_raw = std::numeric_limits<double>::quiet_NaN();
 };
+
+// Writes one model's feature row (row_width values) for a surrogate.
+typedef void (*RowEncoder)(const Model &m, int n_morphogens, uint16_t *row);
 
 class Search {
  public:
@@ -82,6 +89,20 @@ class Search {
   // ties with its parent and changes nothing.  chosen (optional) receives
   // the kept candidates' indices, in pending order.
   void screen(const double *scores, std::vector<size_t> *chosen = nullptr);
+  // Fraction of each island's slots that screen() sends to the simulator
+  // (1 = all): the slots whose kept candidate promises the largest gain over
+  // its parent (score minus the parent's log_raw; parents without one rank
+  // last).  A slot left out keeps its parent, as a child that lost would.
+  void set_screen_fraction(double f) { screen_fraction_ = f; }
+  double screen_fraction() const { return screen_fraction_; }
+  // For the next advance() only: while breeding k > 1 candidates per slot,
+  // each island's thread also writes every candidate's feature row to
+  // rows + i * row_width (i in candidates() order), if they fit in cap rows.
+  void set_candidate_rows(uint16_t *rows, size_t cap, int row_width, RowEncoder enc) {
+    rows_ = rows, rows_cap_ = cap, row_width_ = row_width, encoder_ = enc;
+  }
+  // True when the last advance() wrote those rows.
+  bool candidate_rows_written() const { return rows_written_; }
 
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
@@ -106,7 +127,7 @@ class Search {
   // unique across islands while islands breed concurrently.
   int new_label(int island) { return 11 + island + p_.n_islands * label_next_[island]++; }
   void prepare(Individual *ind, int island);
-  void reproduce(int island);
+  void reproduce(int island, uint16_t *rows);
   void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2, int island);
   void choose_next_generation(int island, std::vector<Individual *> *winners);
   void migrate();
@@ -124,6 +145,12 @@ class Search {
   std::vector<std::vector<Individual>> cand_;  // oversample x island size per island
   std::vector<Individual *> cand_ptrs_;
   int oversample_ = 1;
+  double screen_fraction_ = 1.0;
+  uint16_t *rows_ = nullptr;  // candidate feature rows (set_candidate_rows)
+  size_t rows_cap_ = 0;
+  int row_width_ = 0;
+  RowEncoder encoder_ = nullptr;
+  bool rows_written_ = false;
   std::vector<std::vector<int>> perm_;  // shuffled parent order per island
   std::vector<Individual *> pending_;
   std::vector<Individual> pareto_;

@@ -87,6 +87,7 @@ Surrogate::~Surrogate() {
     for (void *b : lb)
       if (b) (void)hipFree(b);
   }
+  if (h_rows_) (void)hipHostFree(h_rows_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -258,17 +259,22 @@ int Surrogate::add_rows(int n, const uint16_t *rows, const double *log_errors) {
   return 0;
 }
 
+void encode_model_bf16(const Model &m, int n_morphogens, uint16_t *row) {
+  float f[MLP_WIDTH];
+  encode_model(m, n_morphogens, f);
+  for (int k = 0; k < MLP_WIDTH; ++k) row[k] = float_to_bf16(f[k]);
+}
+
 void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vector<uint16_t> *rows) const {
+  rows->assign(models.size() * MLP_WIDTH, 0);
+  encode_rows(models, rows->data());
+}
+
+void Surrogate::encode_rows(const std::vector<const Model *> &models, uint16_t *rows) const {
   const size_t n = models.size();
-  rows->assign(n * MLP_WIDTH, 0);
   const int n_threads = (int)std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), (n + 255) / 256));
   auto work = [&](int t) {
-    float f[MLP_WIDTH];
-    for (size_t i = t; i < n; i += n_threads) {
-      encode_model(*models[i], p_.n_morphogens, f);
-      uint16_t *r = rows->data() + i * MLP_WIDTH;
-      for (int k = 0; k < MLP_FEATURES; ++k) r[k] = float_to_bf16(f[k]);
-    }
+    for (size_t i = t; i < n; i += n_threads) encode_model_bf16(*models[i], p_.n_morphogens, rows + i * MLP_WIDTH);
   };
   if (n_threads == 1) {
     work(0);
@@ -277,6 +283,21 @@ void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vecto
   std::vector<std::thread> pool;
   for (int t = 0; t < n_threads; ++t) pool.emplace_back(work, t);
   for (std::thread &th : pool) th.join();
+}
+
+uint16_t *Surrogate::staging_rows(size_t n) {
+  if (n > h_rows_cap_) {
+    if (h_rows_) (void)hipHostFree(h_rows_);
+    h_rows_ = nullptr;
+    h_rows_cap_ = 0;
+    if (hipHostMalloc((void **)&h_rows_, sizeof(uint16_t) * n * MLP_WIDTH, hipHostMallocDefault) != hipSuccess) {
+      h_rows_ = nullptr;
+      set_last_error(GRN_E_NOMEM, "hipHostMalloc of %zu staging rows failed", n);
+      return nullptr;
+    }
+    h_rows_cap_ = n;
+  }
+  return h_rows_;
 }
 
 int Surrogate::train(int steps, double *last_loss) {
@@ -354,7 +375,10 @@ int Surrogate::predict(int n, const float *features, double *log_errors) {
 int Surrogate::predict_rows(int n, const uint16_t *rows, double *log_errors) {
   if (int rc0 = finish()) return rc0;
   SUR_TRY(hipEventRecord(ev0_, stream_));
-  std::vector<float> y;
+  // chunks of rows_cap_ rows follow one another on the stream (each upload
+  // waits for the previous chunk's forward pass); one wait at the end
+  std::vector<float> &y = host_y_;  // a member: outlives an early return with copies in flight
+  y.resize(std::max(n, 1));
   for (int b = 0; b < n; b += rows_cap_) {
     const int k = std::min(rows_cap_, n - b), kp = round_rows(k);
     SUR_TRY(hipMemcpyAsync(d_x_, rows + (size_t)b * MLP_WIDTH, sizeof(uint16_t) * (size_t)k * MLP_WIDTH,
@@ -367,13 +391,11 @@ int Surrogate::predict_rows(int n, const uint16_t *rows, double *log_errors) {
       (void)hipStreamSynchronize(stream_);
       return rc;
     }
-    y.resize(k);
-    SUR_TRY(hipMemcpyAsync(y.data(), d_y_, sizeof(float) * k, hipMemcpyDeviceToHost, stream_));
-    SUR_TRY(hipStreamSynchronize(stream_));
-    for (int i = 0; i < k; ++i) log_errors[b + i] = (double)y[i] * sigma_ + mu_;
+    SUR_TRY(hipMemcpyAsync(y.data() + b, d_y_, sizeof(float) * k, hipMemcpyDeviceToHost, stream_));
   }
   SUR_TRY(hipEventRecord(ev1_, stream_));
-  SUR_TRY(hipEventSynchronize(ev1_));
+  SUR_TRY(hipStreamSynchronize(stream_));
+  for (int i = 0; i < n; ++i) log_errors[i] = (double)y[i] * sigma_ + mu_;
   float ms = 0.f;
   SUR_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
   stats_.predict_ms = ms;

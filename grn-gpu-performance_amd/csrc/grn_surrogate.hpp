@@ -25,6 +25,9 @@ void encode_model(const Model &m, int n_morphogens, float *out);
 // Round to nearest even, the conversion the device uses for float -> bf16.
 uint16_t float_to_bf16(float f);
 
+// encode_model straight into one BF16 row of MLP_WIDTH values.
+void encode_model_bf16(const Model &m, int n_morphogens, uint16_t *row);
+
 class Surrogate {
  public:
   ~Surrogate();
@@ -43,6 +46,11 @@ class Surrogate {
   int predict_rows(int n, const uint16_t *rows, double *log_errors);
   // Encodes models straight into BF16 rows (n x MLP_WIDTH), on host threads.
   void encode_rows(const std::vector<const Model *> &models, std::vector<uint16_t> *rows) const;
+  void encode_rows(const std::vector<const Model *> &models, uint16_t *rows) const;
+  // A pinned host buffer of at least n rows (MLP_WIDTH BF16 values each) for
+  // feature uploads, kept until the surrogate is destroyed; n = 0 returns the
+  // current buffer.  Null (with the error set) if the allocation fails.
+  uint16_t *staging_rows(size_t n);
   // One forward and backward pass on given rows (standardised targets), no
   // update: loss and the gradient of every parameter.
   int gradient(int n, const float *features, const double *targets, float *grad, double *loss);
@@ -81,6 +89,9 @@ class Surrogate {
   int *d_idx_ = nullptr;
   size_t idx_cap_ = 0;
   std::vector<int> host_idx_;  // source of the pending index upload
+  uint16_t *h_rows_ = nullptr;  // pinned staging rows
+  std::vector<float> host_y_;   // predictions copied back
+  size_t h_rows_cap_ = 0;
   bool training_ = false;      // steps enqueued, finish() not yet called
   uint64_t rng_ = 0;
   long adam_t_ = 0;

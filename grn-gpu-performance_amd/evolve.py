@@ -94,7 +94,8 @@ class Run:
 
 
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
-           train_steps: int = 20, surrogate_batch: int = 4096, progress=None) -> Run:
+           train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
+           progress=None) -> Run:
     if prob.target is None:
         make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
@@ -105,6 +106,7 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], batch=surrogate_batch,
                             seed=prob.search["seed"])
         search.set_oversample(oversample)
+        search.set_screen_fraction(screen_fraction)
     run = Run(screened)
     t0 = time.perf_counter()
     for gen in range(max_generations):

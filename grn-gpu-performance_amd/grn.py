@@ -45,7 +45,7 @@ EXPORTED = [
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
     "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
-    "grn_search_run_generation_screened",
+    "grn_search_set_screen_fraction", "grn_search_record_raw", "grn_search_run_generation_screened",
     "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
     "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
     "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
@@ -230,6 +230,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_set_oversample.argtypes = [vp, ctypes.c_int]
     lib.grn_search_candidates.argtypes = [vp, ip, ctypes.POINTER(vp)]
     lib.grn_search_screen.argtypes = [vp, vp]
+    lib.grn_search_set_screen_fraction.argtypes = [vp, ctypes.c_double]
+    lib.grn_search_record_raw.argtypes = [vp, vp]
     lib.grn_search_run_generation_screened.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                        ctypes.POINTER(_GenerationStats),
                                                        ctypes.POINTER(_ScreenStats)]
@@ -538,6 +540,11 @@ class Search:
         except Exception:
             pass
 
+    def n_pending(self) -> int:
+        n = ctypes.c_int()
+        _check(self._lib.grn_search_pending(self._h, ctypes.byref(n), None, None, None, None))
+        return n.value
+
     def pending(self):
         """(models, datasets, max_errors, dt) of the individuals awaiting an error."""
         n = ctypes.c_int()
@@ -578,6 +585,19 @@ class Search:
     def screen(self, scores):
         sc = np.ascontiguousarray(scores, dtype=np.float64)
         _check(self._lib.grn_search_screen(self._h, sc.ctypes.data))
+
+    def set_screen_fraction(self, fraction: float):
+        """Fraction of each island's slots screen() sends to the simulator:
+        those whose kept candidate's score most undercuts the parent's
+        recorded ln raw error (record_raw); the rest keep their parents."""
+        _check(self._lib.grn_search_set_screen_fraction(self._h, float(fraction)))
+
+    def record_raw(self, raw_errors):
+        """Raw errors of the pending models (before advance)."""
+        r = np.ascontiguousarray(raw_errors, dtype=np.float64)
+        if r.shape != (self.n_pending(),):
+            raise ValueError(f"expected {self.n_pending()} raw errors")
+        _check(self._lib.grn_search_record_raw(self._h, r.ctypes.data if r.size else None))
 
     def run_generation_screened(self, ctx: "Context", surrogate: "Surrogate", train_steps: int = 50,
                                 path: int = PATH_AUTO):

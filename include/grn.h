@@ -425,6 +425,15 @@ int grn_search_set_oversample(void *search, int k);
 int grn_search_candidates(void *search, int *n, const grn_model **models);
 /* Scores (lower is better) of the candidates, in order; fills pending. */
 int grn_search_screen(void *search, const double *scores);
+/* Fraction (0, 1] of each island's slots that grn_search_screen sends to the
+ * simulator: those whose kept candidate's score most undercuts the parent's
+ * recorded ln raw error (grn_search_record_raw).  The other slots keep their
+ * parent, as a losing child would.  Default 1. */
+int grn_search_set_screen_fraction(void *search, double fraction);
+/* Records the raw errors of the pending models (call before
+ * grn_search_advance): their ln is what grn_search_set_screen_fraction
+ * compares a child's score with. */
+int grn_search_record_raw(void *search, const double *raw_errors);
 /* One screened generation: predict the candidates, keep the best per slot
  * (passing over clones of the parent), simulate them on ctx, add them to the
  * surrogate's samples, enqueue train_steps Adam steps -- they run on the

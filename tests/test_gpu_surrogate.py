@@ -257,3 +257,29 @@ def test_screened_generations_report_and_keep_simulated_errors():
     assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
     sur.close()
     ctx.close()
+
+
+def test_screened_generations_with_a_screen_fraction():
+    """With screen fraction f only ceil(f n) slots per island are simulated;
+    the best simulated error never rises and the reported best re-simulates
+    to the same error."""
+    ctx = grn.Context(0)
+    ds = search_problem(ctx, seed=6)
+    s = grn.Search(island_datasets=[ds, ds], n_islands=2, island_size=128, seed=4, **SEARCH)
+    sur = grn.Surrogate(0, n_morphogens=3, batch=512, capacity=1 << 14, seed=5)
+    s.set_oversample(4)
+    s.set_screen_fraction(0.25)
+    g, sc = s.run_generation_screened(ctx, sur, train_steps=50)  # initial population, all simulated
+    assert sc["n_simulated"] == 256
+    prev = g["min_fit"]
+    for gen in range(4):
+        print(f"[gpu] screened generation {gen}, fraction 0.25", flush=True)
+        g, sc = s.run_generation_screened(ctx, sur, train_steps=50)
+        assert sc["n_candidates"] == 4 * 256 and sc["n_simulated"] == 2 * 32
+        assert g["n_evaluated"] == 64
+        assert g["min_fit"] <= prev
+        prev = g["min_fit"]
+    best, err = s.best()
+    assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
+    sur.close()
+    ctx.close()

@@ -245,3 +245,53 @@ def test_search_driven_by_the_oracle():
     best, err = s.best()
     r, _ = ora.evaluate(p, init, target, best.genes, best.links)
     assert r["fitness"] == err
+
+
+def test_screen_fraction_keeps_the_slots_that_promise_most():
+    """grn_search_set_screen_fraction: per island, only the ceil(f n) slots
+    whose kept candidate most undercuts the parent's recorded ln raw error are
+    simulated; the others keep their parents."""
+    n_isl, size, k, f = 2, 40, 3, 0.25
+    s = grn.Search(n_islands=n_isl, island_size=size, seed=5)
+    models, ds, me, dt = s.pending()
+    errs = np.array([fake_error(m) for m in models])
+    s.record_raw(np.ones(len(models)))  # every parent: ln raw error 0
+    s.set_oversample(k)
+    s.set_screen_fraction(f)
+    s.advance(errs)
+    cands = s.candidates()
+    assert len(cands) == n_isl * k * size
+    # one score per slot, shared by the slot's k candidates (order: island,
+    # repetition, slot), so the kept candidate's score is the slot's value
+    rng = np.random.default_rng(1)
+    v = rng.permutation(n_isl * size).reshape(n_isl, size).astype(float)
+    scores = np.repeat(v[:, None, :], k, axis=1).ravel()
+    s.screen(scores)
+    picked = [m.to_text(17) for m in s.pending()[0]]
+    keep = int(np.ceil(f * size))
+    assert len(picked) == n_isl * keep
+    texts = np.array([m.to_text(17) for m in cands], dtype=object).reshape(n_isl, k, size)
+    at = 0
+    for isl in range(n_isl):
+        slots = np.sort(np.argsort(v[isl], kind="stable")[:keep])  # pending keeps slot order
+        for slot in slots:
+            assert picked[at] in set(texts[isl, :, slot])
+            at += 1
+    # the unsimulated slots cannot replace their parents: the search goes on
+    st = s.advance(np.array([fake_error(m) for m in s.pending()[0]]))
+    assert st["n_evaluated"] == n_isl * keep
+    assert len(s.candidates()) == n_isl * k * size
+
+
+def test_screen_fraction_without_recorded_errors_and_bad_values():
+    s = grn.Search(n_islands=1, island_size=20, seed=2)
+    s.set_oversample(2)
+    s.set_screen_fraction(0.5)
+    s.advance(np.array([fake_error(m) for m in s.pending()[0]]))
+    s.screen(np.zeros(len(s.candidates())))
+    assert len(s.pending()[0]) == 10  # no parent has a recorded error: any 10 slots
+    for bad in (0.0, -0.5, 1.5, float("nan")):
+        with pytest.raises(grn.GrnError):
+            s.set_screen_fraction(bad)
+    with pytest.raises(ValueError):
+        s.record_raw(np.ones(3))
