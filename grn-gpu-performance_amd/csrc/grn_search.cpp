@@ -2,7 +2,10 @@
 // the simulation path); see grn_search.hpp for the reference map.
 #include "grn_search.hpp"
 
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -289,6 +292,12 @@ void Search::cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model 
 }
 
 void Search::mutate(Rng &rng, Model *m, int island) {
+  LabelStream ls{island, label_next_[island], 1};
+  mutate(rng, m, &ls);
+  label_next_[island] += ls.used;
+}
+
+void Search::mutate(Rng &rng, Model *m, LabelStream *labels) {
   // Model::mutate (model.cpp:499-547)
   const int n_min = n_morph_;
   for (int i = (int)m->genes.size() - 1; i >= 0; --i) {
@@ -297,7 +306,7 @@ void Search::mutate(Rng &rng, Model *m, int island) {
       // duplicateProduct (model.cpp:317-333)
       Gene g = m->genes[i];
       g.init = 0;
-      g.label = new_label(island);
+      g.label = new_label(labels);
       m->genes.push_back(g);
       const int regulated = m->genes[rng.below((int)m->genes.size())].label;
       remove_link(m, g.label, regulated);
@@ -334,6 +343,31 @@ void Search::mutate(Rng &rng, Model *m, int island) {
 
 // ---- the search ---------------------------------------------------------------
 
+// Threads this process may run on (its affinity mask), at least 1.
+static int host_threads() {
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof set, &set) == 0) return std::max(1, CPU_COUNT(&set));
+  return std::max(1u, std::thread::hardware_concurrency());
+}
+
+// f(0) ... f(n - 1) on up to host_threads() threads, dynamically scheduled.
+template <class F>
+static void parallel_for(int n, F f) {
+  const int n_threads = std::min(n, host_threads());
+  if (n_threads <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto work = [&] {
+    for (int i; (i = next.fetch_add(1)) < n;) f(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < n_threads; ++t) pool.emplace_back(work);
+  work();
+  for (std::thread &th : pool) th.join();
+}
+
 Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
   if (p_.island_size % 2) p_.island_size++;  // searchalgodetcrowd.cpp:33-35
   n_morph_ = p_.n_morphogens;
@@ -344,6 +378,8 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
   label_next_.assign(p_.n_islands, 0);
   id_next_.assign(p_.n_islands, 0);
   perm_.resize(p_.n_islands);
+  round_seed_.resize(p_.n_islands);
+  round_labels_.resize(p_.n_islands);
   for (int is = 0; is < p_.n_islands; ++is) {
     Rng &rng = rngs_[is];
     perm_[is].resize(p_.island_size);
@@ -355,7 +391,7 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
       Individual ind;
       ind.model = create_random(rng, lo + rng.below(hi - lo + 1));
       ind.dataset = island_datasets ? island_datasets[is] : 0;
-      prepare(&ind, is);
+      prepare(&ind, is, ++id_next_[is]);
       ind.complexity = ind.model.complexity_in_use(n_morph_);
       islands_[is].push_back(std::move(ind));
     }
@@ -364,8 +400,8 @@ Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
     for (Individual &ind : isl) pending_.push_back(&ind);
 }
 
-void Search::prepare(Individual *ind, int island) {
-  ind->id = ((uint64_t)island << 40) | ++id_next_[island];
+void Search::prepare(Individual *ind, int island, uint64_t serial) {
+  ind->id = ((uint64_t)island << 40) | serial;
   ind->error = -1;
   // the in-use complexity is computed once an individual is pending
   // (a screened-out candidate never needs it)
@@ -388,7 +424,7 @@ static void copy_model(const Model &from, Model *to) {
 }
 
 void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2,
-                        int island) {
+                        LabelStream *labels, uint64_t serial1, uint64_t serial2) {
   // one iteration of SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:70-97)
   if (rng.below(100) < (int)std::lround(p_.crossover_rate * 100)) {
     cross(rng, a.model, b.model, &k1->model, &k2->model);
@@ -396,17 +432,17 @@ void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Indi
     copy_model(a.model, &k1->model);
     copy_model(b.model, &k2->model);
   }
-  mutate(rng, &k1->model, island);
-  mutate(rng, &k2->model, island);
+  mutate(rng, &k1->model, labels);
+  mutate(rng, &k2->model, labels);
   k1->parent_error = a.error;
   k2->parent_error = b.error;
   k1->dataset = a.dataset;
   k2->dataset = b.dataset;
-  prepare(k1, island);
-  prepare(k2, island);
+  prepare(k1, labels->island, serial1);
+  prepare(k2, labels->island, serial2);
 }
 
-void Search::reproduce(int is, uint16_t *rows) {
+void Search::reproduce(int is) {
   // SearchAlgoDetCrowd::reproduce (searchalgodetcrowd.cpp:63-101): shuffle
   // the parents, breed consecutive pairs; child i belongs to parent perm[i]
   Rng &rng = rngs_[is];
@@ -417,19 +453,39 @@ void Search::reproduce(int is, uint16_t *rows) {
   std::vector<Individual> &kids = children_[is];
   kids.assign(n, Individual());
   if (oversample_ <= 1) {
-    for (int i = 0; i + 1 < n; i += 2)
-      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1], is);
+    LabelStream ls{is, label_next_[is], 1};
+    for (int i = 0; i + 1 < n; i += 2) {
+      const uint64_t s1 = ++id_next_[is], s2 = ++id_next_[is];
+      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &kids[i], &kids[i + 1], &ls, s1, s2);
+    }
+    label_next_[is] += ls.used;
     for (Individual &k : kids) k.complexity = k.model.complexity_in_use(n_morph_);
     return;
   }
-  // k candidate pairs per parent pair, all with the same parents
+  // k candidate rounds, all with the same parent pairs; round r draws from a
+  // stream seeded by the island's r-th draw here, so the rounds can breed
+  // concurrently (breed_round) with results independent of the schedule
+  round_seed_[is].resize(oversample_);
+  for (int r = 0; r < oversample_; ++r) round_seed_[is][r] = rng.next();
+  round_labels_[is].assign(oversample_, 0);
+  cand_[is].assign((size_t)oversample_ * n, Individual());
+}
+
+void Search::breed_round(int is, int r, uint16_t *rows) {
+  Rng rng(round_seed_[is][r]);
+  const std::vector<int> &perm = perm_[is];
+  const std::vector<Individual> &pop = islands_[is];
+  const int n = (int)pop.size();
   std::vector<Individual> &cand = cand_[is];
-  cand.assign((size_t)oversample_ * n, Individual());
-  for (int r = 0; r < oversample_; ++r)
-    for (int i = 0; i + 1 < n; i += 2)
-      breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1], is);
+  LabelStream ls{is, label_next_[is] + r, oversample_};
+  const uint64_t base = id_next_[is] + (uint64_t)r * n;
+  for (int i = 0; i + 1 < n; i += 2)
+    breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1], &ls,
+               base + i + 1, base + i + 2);
+  round_labels_[is][r] = ls.used;
   if (rows)
-    for (size_t c = 0; c < cand.size(); ++c) encoder_(cand[c].model, n_morph_, rows + c * (size_t)row_width_);
+    for (size_t c = (size_t)r * n; c < (size_t)(r + 1) * n; ++c)
+      encoder_(cand[c].model, n_morph_, rows + c * (size_t)row_width_);
 }
 
 static bool same_model(const Model &a, const Model &b) {
@@ -448,37 +504,36 @@ static bool same_model(const Model &a, const Model &b) {
 }
 
 void Search::screen(const double *scores, std::vector<size_t> *chosen) {
-  size_t off = 0;
-  pending_.clear();
-  if (chosen) chosen->clear();
-  std::vector<int> pick;
-  std::vector<double> gain;
-  std::vector<int> order;
-  for (int is = 0; is < p_.n_islands; ++is) {
+  // candidate offsets per island (island-major order)
+  std::vector<size_t> off(p_.n_islands + 1, 0);
+  for (int is = 0; is < p_.n_islands; ++is) off[is + 1] = off[is] + cand_[is].size();
+  std::vector<std::vector<size_t>> kept(p_.n_islands);  // per island, in slot order
+  parallel_for(p_.n_islands, [&](int is) {
     std::vector<Individual> &cand = cand_[is];
     std::vector<Individual> &kids = children_[is];
     const std::vector<Individual> &pop = islands_[is];
     const std::vector<int> &perm = perm_[is];
+    const double *sc = scores + off[is];
     const int n = (int)kids.size();
-    pick.assign(n, 0);
-    gain.assign(n, 0.0);
+    std::vector<int> pick(n);
+    std::vector<double> gain(n);
     for (int i = 0; i < n; ++i) {
       const Individual &parent = pop[perm[i]];
       int best = -1, best_any = 0;
       for (int r = 0; r < oversample_; ++r) {
-        const double sc = scores[off + (size_t)r * n + i];
-        if (sc < scores[off + (size_t)best_any * n + i]) best_any = r;
+        const double v = sc[(size_t)r * n + i];
+        if (v < sc[(size_t)best_any * n + i]) best_any = r;
         if (same_model(cand[(size_t)r * n + i].model, parent.model)) continue;
-        if (best < 0 || sc < scores[off + (size_t)best * n + i]) best = r;
+        if (best < 0 || v < sc[(size_t)best * n + i]) best = r;
       }
       pick[i] = best < 0 ? best_any : best;
-      const double g = scores[off + (size_t)pick[i] * n + i] - parent.log_raw;
+      const double g = sc[(size_t)pick[i] * n + i] - parent.log_raw;
       gain[i] = std::isnan(g) ? std::numeric_limits<double>::infinity() : g;
     }
     // the slots to simulate, in slot order
     int keep = n;
     if (screen_fraction_ < 1.0) keep = std::max(1, std::min(n, (int)std::ceil(screen_fraction_ * n)));
-    order.resize(n);
+    std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     if (keep < n) {
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return gain[a] < gain[b]; });
@@ -489,18 +544,25 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
     for (int i = 0; i < n; ++i) {
       if (next < order.size() && order[next] == i) {
         ++next;
-        const size_t c = off + (size_t)pick[i] * n + i;
-        if (chosen) chosen->push_back(c);
-        kids[i] = std::move(cand[c - off]);
+        const size_t c = (size_t)pick[i] * n + i;
+        kept[is].push_back(c);
+        kids[i] = std::move(cand[c]);
         kids[i].complexity = kids[i].model.complexity_in_use(n_morph_);
-        pending_.push_back(&kids[i]);
       } else {
         // not simulated: cannot replace its parent
         kids[i].error = std::numeric_limits<double>::infinity();
       }
     }
-    off += cand.size();
     cand.clear();
+  });
+  pending_.clear();
+  if (chosen) chosen->clear();
+  for (int is = 0; is < p_.n_islands; ++is) {
+    const size_t n = children_[is].size();
+    for (size_t c : kept[is]) {
+      if (chosen) chosen->push_back(off[is] + c);
+      pending_.push_back(&children_[is][c % n]);
+    }
   }
   cand_ptrs_.clear();
   rows_written_ = false;
@@ -652,17 +714,16 @@ void Search::advance(const double *errors) {
   }
   rows_ = nullptr;  // the buffer is lent for this call only
   // islands breed independently (own random stream, label stream and
-  // individuals), so they breed concurrently with identical results
-  const int n_threads = std::max(1, std::min<int>(p_.n_islands, (int)std::thread::hardware_concurrency()));
-  if (n_threads > 1) {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < n_threads; ++t)
-      pool.emplace_back([this, t, n_threads, &rows] {
-        for (int is = t; is < p_.n_islands; is += n_threads) reproduce(is, rows[is]);
-      });
-    for (std::thread &th : pool) th.join();
-  } else {
-    for (int is = 0; is < p_.n_islands; ++is) reproduce(is, rows[is]);
+  // individuals), so they breed concurrently with identical results; so do
+  // the candidate rounds of one island
+  parallel_for(p_.n_islands, [this](int is) { reproduce(is); });
+  if (oversample_ > 1) {
+    const int K = oversample_;
+    parallel_for(p_.n_islands * K, [this, K, &rows](int t) { breed_round(t / K, t % K, rows[t / K]); });
+    for (int is = 0; is < p_.n_islands; ++is) {
+      label_next_[is] += K * *std::max_element(round_labels_[is].begin(), round_labels_[is].end());
+      id_next_[is] += (uint64_t)K * islands_[is].size();
+    }
   }
   for (int is = 0; is < p_.n_islands; ++is) {
     if (oversample_ <= 1)

@@ -113,8 +113,17 @@ class Search {
   // Operators, public for tests.
   Model create_random(Rng &rng, int n_products);
   void cross(Rng &rng, const Model &p1, const Model &p2, Model *c1, Model *c2);
+  // Names for duplicated products: island i hands out 11 + i + n_islands * k
+  // for k = first, first + stride, first + 2 stride, ...
+  struct LabelStream {
+    int island;
+    long first;
+    int stride;
+    int used = 0;
+  };
   // island: whose label stream names duplicated products
   void mutate(Rng &rng, Model *m, int island = 0);
+  void mutate(Rng &rng, Model *m, LabelStream *labels);
   void remove_excess(Rng &rng, Model *m);
 
  private:
@@ -124,11 +133,22 @@ class Search {
   void mutate_link_params(Rng &rng, Link *l);
   // Model::createNewLabel (model.cpp:373-379) hands out 11, 12, ... from one
   // counter; here island i draws 11 + i + n_islands * k, so labels stay
-  // unique across islands while islands breed concurrently.
-  int new_label(int island) { return 11 + island + p_.n_islands * label_next_[island]++; }
-  void prepare(Individual *ind, int island);
-  void reproduce(int island, uint16_t *rows);
-  void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2, int island);
+  // unique across islands while islands breed concurrently.  The k of one
+  // island come from label_next_[i] on: in order when the island breeds its
+  // children (oversample 1); when it breeds k > 1 candidate rounds at once,
+  // round r takes every k-th value from label_next_[i] + r, so the rounds
+  // never share a label either.
+  int new_label(LabelStream *ls) {
+    return (int)(11 + ls->island + (long)p_.n_islands * (ls->first + (long)ls->stride * ls->used++));
+  }
+  void prepare(Individual *ind, int island, uint64_t serial);
+  // Shuffles the island's parents; with oversample 1 breeds its children.
+  void reproduce(int island);
+  // Candidate round r of the island (oversample > 1): its own random stream
+  // and label stream; rows (optional) receive the round's feature rows.
+  void breed_round(int island, int r, uint16_t *rows);
+  void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2,
+                  LabelStream *labels, uint64_t serial1, uint64_t serial2);
   void choose_next_generation(int island, std::vector<Individual *> *winners);
   void migrate();
   void update_pareto(const std::vector<Individual *> &fresh);
@@ -152,6 +172,8 @@ class Search {
   RowEncoder encoder_ = nullptr;
   bool rows_written_ = false;
   std::vector<std::vector<int>> perm_;  // shuffled parent order per island
+  std::vector<std::vector<uint64_t>> round_seed_;  // per island and candidate round
+  std::vector<std::vector<int>> round_labels_;     // labels used per island and round
   std::vector<Individual *> pending_;
   std::vector<Individual> pareto_;
   grn_generation_stats stats_{};
