@@ -140,60 +140,74 @@ def bench_islands(args):
 
 
 def bench_time_to_fitness(args):
-    """Plain batched vs surrogate-screened evolution on the same problem and
+    """Plain batched vs surrogate-screened evolution on the same problems and
     seeds: wall time until the simulator-verified best error reaches each
-    threshold (the plain run's final best among them)."""
+    threshold.  One seed is one problem (its own blob inputs and hidden-GRN
+    target) and one search seed; single runs vary a lot, so the value is the
+    median over the seeds of plain time / screened time to the plain run's
+    final best (0 where the screened run never reached it)."""
     import grn
     import evolve
 
+    seeds = [int(x) for x in args.seeds.split(",")] if args.seeds else [args.seed]
     ctx = grn.Context(0)
-    prob = evolve.scaled_config4(seed=args.seed)
-    evolve.make_target(ctx, prob)
-    runs = {}
-    for name, screened in (("plain", False), ("screened", True)):
-        log(f"[bench] {name} evolution, {args.generations} generations")
-        runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-3,
-                                   oversample=args.oversample, train_steps=args.train_steps,
-                                   screen_fraction=args.screen_fraction, progress=log)
-    plain, scr = runs["plain"], runs["screened"]
-    thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
-    table = []
-    for thr in thresholds:
-        tp, ts = plain.time_to(thr), scr.time_to(thr)
-        table.append({"threshold": thr, "plain_s": tp, "screened_s": ts,
-                      "speedup": (tp / ts) if tp and ts else None})
-    sp = [s["spearman"] for s in scr.screen if s["n_candidates"]]
-    mae = [s["mae"] for s in scr.screen if s["n_candidates"]]
-    target = plain.best[-1]
-    tp, ts = plain.time_to(target), scr.time_to(target)
+    per_seed, ratios, sp, mae, phase = [], [], [], [], {"predict_ms": [], "simulate_ms": [], "breed_ms": [],
+                                                        "train_ms": []}
+    gen_ms = {"plain": [], "screened": []}
+    for seed in seeds:
+        prob = evolve.scaled_config4(seed=seed)
+        evolve.make_target(ctx, prob)
+        runs = {}
+        for name, screened in (("plain", False), ("screened", True)):
+            log(f"[bench] seed {seed}: {name} evolution, {args.generations} generations")
+            runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-4,
+                                       oversample=args.oversample, train_steps=args.train_steps,
+                                       screen_fraction=args.screen_fraction, screen_policy=args.screen_policy,
+                                       progress=log)
+        plain, scr = runs["plain"], runs["screened"]
+        thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
+        table = []
+        for thr in thresholds:
+            tp, ts = plain.time_to(thr), scr.time_to(thr)
+            table.append({"threshold": thr, "plain_s": tp, "screened_s": ts,
+                          "speedup": (tp / ts) if tp and ts else None})
+        tp, ts = plain.time_to(plain.best[-1]), scr.time_to(plain.best[-1])
+        ratios.append((tp / ts) if tp and ts else 0.0)
+        sp += [x["spearman"] for x in scr.screen if x["n_candidates"]]
+        mae += [x["mae"] for x in scr.screen if x["n_candidates"]]
+        for k in phase:
+            phase[k] += [x[k] for x in scr.screen[2:]]
+        gen_ms["plain"].append(plain.wall[-1] / len(plain.wall) * 1e3)
+        gen_ms["screened"].append(scr.wall[-1] / len(scr.wall) * 1e3)
+        per_seed.append({"seed": seed, "thresholds": table,
+                         "plain": {"generations": len(plain.best), "wall_s": plain.wall[-1], "best": plain.best[-1],
+                                   "verified": plain.verified},
+                         "screened": {"generations": len(scr.best), "wall_s": scr.wall[-1], "best": scr.best[-1],
+                                      "verified": scr.verified}})
     out = {
         "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
-        "value": (tp / ts) if tp and ts else None,
-        "unit": "x (plain wall time / screened wall time to the plain run's final simulator-verified best)",
+        "value": float(np.median(ratios)),
+        "unit": "x (median over seeds of plain wall time / screened wall time to the plain run's final "
+                "simulator-verified best)",
         "n_gpus": 1,
         "steps": args.generations,
         "warmup": 0,
-        "ms_per_step": plain.wall[-1] / len(plain.wall) * 1e3,
+        "ms_per_step": float(np.mean(gen_ms["plain"])),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 simulation, bf16/fp32 surrogate",
         "data": "synthetic",
-        "config": {"workload": "time to fitness, config 4 at reduced size: " + prob.name,
-                   "oversample": args.oversample, "screen_fraction": args.screen_fraction,
-                   "train_steps_per_generation": args.train_steps,
-                   "seed": args.seed},
-        "thresholds": table,
-        "plain": {"generations": len(plain.best), "wall_s": plain.wall[-1], "best": plain.best[-1],
-                  "cell_steps_per_s": plain.cell_steps / plain.wall[-1], "verified": plain.verified},
-        "screened": {"generations": len(scr.best), "wall_s": scr.wall[-1], "best": scr.best[-1],
-                     "cell_steps_per_s": scr.cell_steps / scr.wall[-1], "verified": scr.verified,
-                     "spearman_mean": float(np.mean(sp)) if sp else None,
-                     "spearman_last": sp[-1] if sp else None,
-                     "mae_mean": float(np.mean(mae)) if mae else None,
-                     "predict_ms_mean": float(np.mean([s["predict_ms"] for s in scr.screen[1:]])),
-                     "train_ms_mean": float(np.mean([s["train_ms"] for s in scr.screen[2:]])),
-                     "breed_ms_mean": float(np.mean([s["breed_ms"] for s in scr.screen[1:]]))},
+        "config": {"workload": "time to fitness, config 4 at reduced size: " + evolve.scaled_config4().name,
+                   "seeds": seeds, "oversample": args.oversample, "screen_fraction": args.screen_fraction,
+                   "screen_policy": ["absolute", "delta"][args.screen_policy],
+                   "train_steps_per_generation": args.train_steps},
+        "speedup_per_seed": ratios,
+        "ms_per_generation": {k: float(np.mean(v)) for k, v in gen_ms.items()},
+        "screened_phase_ms": {k: float(np.mean(v)) for k, v in phase.items()},
+        "spearman_mean": float(np.mean(sp)) if sp else None,
+        "mae_mean": float(np.mean(mae)) if mae else None,
+        "per_seed": per_seed,
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -259,6 +273,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--models", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seeds", default="0,1,2,3", help="evolve: comma-separated problem/search seeds")
+    ap.add_argument("--screen-policy", type=int, default=0, help="evolve: 0 absolute, 1 delta")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["config1", "config2", "evolve", "surrogate"], default="config1",

@@ -436,6 +436,8 @@ void Search::breed_pair(Rng &rng, const Individual &a, const Individual &b, Indi
   mutate(rng, &k2->model, labels);
   k1->parent_error = a.error;
   k2->parent_error = b.error;
+  k1->parent_log_raw = a.log_raw;
+  k2->parent_log_raw = b.log_raw;
   k1->dataset = a.dataset;
   k2->dataset = b.dataset;
   prepare(k1, labels->island, serial1);
@@ -483,9 +485,42 @@ void Search::breed_round(int is, int r, uint16_t *rows) {
     breed_pair(rng, pop[perm[i]], pop[perm[i + 1]], &cand[(size_t)r * n + i], &cand[(size_t)r * n + i + 1], &ls,
                base + i + 1, base + i + 2);
   round_labels_[is][r] = ls.used;
-  if (rows)
+  if (rows) encode_round(is, r, rows);
+}
+
+void Search::encode_round(int is, int r, uint16_t *rows) const {
+  const std::vector<Individual> &cand = cand_[is];
+  const int n = (int)islands_[is].size();
+  if (policy_ != GRN_SCREEN_DELTA) {
     for (size_t c = (size_t)r * n; c < (size_t)(r + 1) * n; ++c)
       encoder_(cand[c].model, n_morph_, rows + c * (size_t)row_width_);
+    return;
+  }
+  float fp[MLP_WIDTH], fc[MLP_WIDTH];
+  for (int i = 0; i < n; ++i) {
+    const Individual &parent = islands_[is][perm_[is][i]];
+    encode_model(parent.model, n_morph_, fp);
+    const size_t c = (size_t)r * n + i;
+    encode_model(cand[c].model, n_morph_, fc);
+    uint16_t *row = rows + c * (size_t)row_width_;
+    for (int k = 0; k < MLP_WIDTH; ++k) row[k] = float_to_bf16(fc[k] - fp[k]);
+    const double lr = std::isnan(parent.log_raw) ? 0.0 : parent.log_raw;
+    row[MLP_FEATURES] = float_to_bf16((float)(lr / 8.0));
+  }
+}
+
+void Search::encode_candidates(uint16_t *rows, int row_width) {
+  if (cand_ptrs_.empty()) return;
+  std::vector<uint16_t *> base(p_.n_islands);
+  size_t at = 0;
+  for (int is = 0; is < p_.n_islands; ++is) {
+    base[is] = rows + at * (size_t)row_width;
+    at += cand_[is].size();
+  }
+  row_width_ = row_width;
+  encoder_ = &encode_model_bf16;
+  const int K = oversample_;
+  parallel_for(p_.n_islands * K, [this, K, &base](int t) { encode_round(t / K, t % K, base[t / K]); });
 }
 
 static bool same_model(const Model &a, const Model &b) {
@@ -527,7 +562,8 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
         if (best < 0 || v < sc[(size_t)best * n + i]) best = r;
       }
       pick[i] = best < 0 ? best_any : best;
-      const double g = sc[(size_t)pick[i] * n + i] - parent.log_raw;
+      const double v = sc[(size_t)pick[i] * n + i];
+      const double g = policy_ == GRN_SCREEN_DELTA ? v : v - parent.log_raw;
       gain[i] = std::isnan(g) ? std::numeric_limits<double>::infinity() : g;
     }
     // the slots to simulate, in slot order
@@ -940,6 +976,15 @@ int grn_search_set_screen_fraction(void *search, double fraction) {
   return 0;
 }
 
+int grn_search_set_screen_policy(void *search, int policy) {
+  SearchHandle *h = as_search(search);
+  if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
+  if (policy != GRN_SCREEN_ABSOLUTE && policy != GRN_SCREEN_DELTA)
+    return grn::set_last_error(GRN_E_INVALID, "unknown screen policy %d", policy);
+  h->s->set_screen_policy(policy);
+  return 0;
+}
+
 int grn_search_record_raw(void *search, const double *raw_errors) {
   SearchHandle *h = as_search(search);
   if (!h || (!raw_errors && h->s->n_pending())) return grn::set_last_error(GRN_E_INVALID, "null argument");
@@ -973,10 +1018,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     } else {
       uint16_t *buf = S->staging_rows(nc);
       if (!buf) return GRN_E_NOMEM;
-      const std::vector<grn::Individual *> &cands = h->s->candidates();
-      std::vector<const grn::Model *> ms(nc);
-      for (int i = 0; i < nc; ++i) ms[i] = &cands[i]->model;
-      S->encode_rows(ms, buf);
+      h->s->encode_candidates(buf, MLP_WIDTH);
       cand_rows = buf;
     }
     std::vector<double> scores(nc);
@@ -1003,12 +1045,20 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     raw[i] = h->results[i].raw_error;
   }
   if ((rc = grn_search_record_raw(search, raw.data()))) return rc;
+  // the surrogate's target per simulated child (NaN: not a sample)
+  const bool delta = h->s->screen_policy() == GRN_SCREEN_DELTA;
+  std::vector<double> target(n, std::numeric_limits<double>::quiet_NaN());
+  for (int i = 0; i < n; ++i) {
+    if (!(raw[i] > 0 && std::isfinite(raw[i]))) continue;
+    const double base = delta ? h->s->pending()[i]->parent_log_raw : 0.0;
+    if (!std::isnan(base)) target[i] = std::log(raw[i]) - base;
+  }
   if (!predicted.empty()) {
     std::vector<double> pv, av;
     for (int i = 0; i < n; ++i)
-      if (raw[i] > 0 && std::isfinite(raw[i])) {
+      if (!std::isnan(target[i])) {
         pv.push_back(predicted[i]);
-        av.push_back(std::log(raw[i]));
+        av.push_back(target[i]);
       }
     sc.spearman = grn::spearman(pv, av);
     double mae = 0;
@@ -1022,9 +1072,9 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     std::vector<int> idx;
     std::vector<double> logs;
     for (int i = 0; i < n; ++i)
-      if (raw[i] > 0 && std::isfinite(raw[i])) {
+      if (!std::isnan(target[i]) && (cand_rows || !delta)) {
         idx.push_back(i);
-        logs.push_back(std::log(raw[i]));
+        logs.push_back(target[i]);
       }
     std::vector<uint16_t> rows(idx.size() * (size_t)MLP_WIDTH);
     if (cand_rows) {
@@ -1039,7 +1089,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   }
   // the training steps run on the device while the host selects and breeds;
   // the next prediction (or any other surrogate call) waits for them
-  if (train_steps > 0 && (rc = S->train_async(train_steps))) return rc;
+  if (train_steps > 0 && S->stats().samples > 0 && (rc = S->train_async(train_steps))) return rc;
   sc.train_ms = S->stats().train_ms;  // device time of the last completed training call
   sc.loss = S->stats().last_loss;     // and its loss
   sc.samples = S->stats().samples;

@@ -61,6 +61,7 @@ struct Individual {
   // ln of the simulated raw error where the caller recorded it (NaN if not):
   // the scale a surrogate predicts on, for comparing a child with its parent
   double log_raw = std::numeric_limits<double>::quiet_NaN();
+  double parent_log_raw = std::numeric_limits<double>::quiet_NaN();  // its slot parent's
 };
 
 // Writes one model's feature row (row_width values) for a surrogate.
@@ -103,6 +104,14 @@ class Search {
   }
   // True when the last advance() wrote those rows.
   bool candidate_rows_written() const { return rows_written_; }
+  // What the candidate rows hold (GRN_SCREEN_*): the candidate's features, or
+  // its features minus its slot parent's with the parent's ln raw error / 8
+  // in the first padding column (a score is then a predicted change of ln
+  // error, and screen() ranks slots by the score alone).
+  void set_screen_policy(int policy) { policy_ = policy; }
+  int screen_policy() const { return policy_; }
+  // Writes every candidate's row as breeding would (rows: candidates() order).
+  void encode_candidates(uint16_t *rows, int row_width);
 
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
@@ -147,6 +156,7 @@ class Search {
   // Candidate round r of the island (oversample > 1): its own random stream
   // and label stream; rows (optional) receive the round's feature rows.
   void breed_round(int island, int r, uint16_t *rows);
+  void encode_round(int island, int r, uint16_t *rows) const;
   void breed_pair(Rng &rng, const Individual &a, const Individual &b, Individual *k1, Individual *k2,
                   LabelStream *labels, uint64_t serial1, uint64_t serial2);
   void choose_next_generation(int island, std::vector<Individual *> *winners);
@@ -166,6 +176,7 @@ class Search {
   std::vector<Individual *> cand_ptrs_;
   int oversample_ = 1;
   double screen_fraction_ = 1.0;
+  int policy_ = GRN_SCREEN_ABSOLUTE;
   uint16_t *rows_ = nullptr;  // candidate feature rows (set_candidate_rows)
   size_t rows_cap_ = 0;
   int row_width_ = 0;

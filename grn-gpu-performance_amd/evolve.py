@@ -95,7 +95,7 @@ class Run:
 
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
-           progress=None) -> Run:
+           screen_policy: int = grn.SCREEN_ABSOLUTE, progress=None) -> Run:
     if prob.target is None:
         make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
@@ -107,6 +107,7 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
                             seed=prob.search["seed"])
         search.set_oversample(oversample)
         search.set_screen_fraction(screen_fraction)
+        search.set_screen_policy(screen_policy)
     run = Run(screened)
     t0 = time.perf_counter()
     for gen in range(max_generations):

@@ -45,7 +45,8 @@ EXPORTED = [
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
     "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
-    "grn_search_set_screen_fraction", "grn_search_record_raw", "grn_search_run_generation_screened",
+    "grn_search_set_screen_fraction", "grn_search_set_screen_policy", "grn_search_record_raw",
+    "grn_search_run_generation_screened",
     "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
     "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
     "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
@@ -57,6 +58,7 @@ MLP_FEATURES = 468
 MLP_WIDTH = 512
 
 MIGRATION_PAIRWISE, MIGRATION_RING, MIGRATION_NONE = 0, 1, 2
+SCREEN_ABSOLUTE, SCREEN_DELTA = 0, 1
 
 
 class GrnError(RuntimeError):
@@ -231,6 +233,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_candidates.argtypes = [vp, ip, ctypes.POINTER(vp)]
     lib.grn_search_screen.argtypes = [vp, vp]
     lib.grn_search_set_screen_fraction.argtypes = [vp, ctypes.c_double]
+    lib.grn_search_set_screen_policy.argtypes = [vp, ctypes.c_int]
     lib.grn_search_record_raw.argtypes = [vp, vp]
     lib.grn_search_run_generation_screened.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                        ctypes.POINTER(_GenerationStats),
@@ -591,6 +594,11 @@ class Search:
         those whose kept candidate's score most undercuts the parent's
         recorded ln raw error (record_raw); the rest keep their parents."""
         _check(self._lib.grn_search_set_screen_fraction(self._h, float(fraction)))
+
+    def set_screen_policy(self, policy: int):
+        """SCREEN_ABSOLUTE (a candidate's ln error) or SCREEN_DELTA (its change
+        of ln error from the slot parent) for run_generation_screened."""
+        _check(self._lib.grn_search_set_screen_policy(self._h, int(policy)))
 
     def record_raw(self, raw_errors):
         """Raw errors of the pending models (before advance)."""

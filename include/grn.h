@@ -430,6 +430,13 @@ int grn_search_screen(void *search, const double *scores);
  * recorded ln raw error (grn_search_record_raw).  The other slots keep their
  * parent, as a losing child would.  Default 1. */
 int grn_search_set_screen_fraction(void *search, double fraction);
+/* What the surrogate scores in grn_search_run_generation_screened:
+ * GRN_SCREEN_ABSOLUTE, a candidate's ln raw error from its own features, or
+ * GRN_SCREEN_DELTA, the change of ln raw error from its slot parent, from the
+ * difference of their features and the parent's error (the surrogate then
+ * learns from children with a recorded parent only).  Default absolute. */
+enum { GRN_SCREEN_ABSOLUTE = 0, GRN_SCREEN_DELTA = 1 };
+int grn_search_set_screen_policy(void *search, int policy);
 /* Records the raw errors of the pending models (call before
  * grn_search_advance): their ln is what grn_search_set_screen_fraction
  * compares a child's score with. */
