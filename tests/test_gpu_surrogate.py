@@ -283,3 +283,33 @@ def test_screened_generations_with_a_screen_fraction():
     assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
     sur.close()
     ctx.close()
+
+
+def test_screened_generations_with_the_delta_policy():
+    """SCREEN_DELTA: the surrogate learns the change of ln error from the slot
+    parent (no samples from the unparented initial population); generations
+    run, the best error never rises and re-simulates to the same value."""
+    ctx = grn.Context(0)
+    ds = search_problem(ctx, seed=7)
+    s = grn.Search(island_datasets=[ds, ds], n_islands=2, island_size=128, seed=6, **SEARCH)
+    sur = grn.Surrogate(0, n_morphogens=3, batch=512, capacity=1 << 14, seed=2)
+    s.set_oversample(4)
+    s.set_screen_policy(grn.SCREEN_DELTA)
+    g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+    assert sc["n_simulated"] == 256 and sc["samples"] == 0  # no parents yet
+    g, sc = s.run_generation_screened(ctx, sur, train_steps=20)  # children of the initial population
+    assert sc["samples"] > 0
+    prev = g["min_fit"]
+    for gen in range(3):
+        print(f"[gpu] delta-screened generation {gen}", flush=True)
+        g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+        assert sc["n_candidates"] == 4 * 256 and sc["n_simulated"] == 256
+        assert -1.0 <= sc["spearman"] <= 1.0 and np.isfinite(sc["mae"])
+        assert g["min_fit"] <= prev
+        prev = g["min_fit"]
+    best, err = s.best()
+    assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
+    with pytest.raises(grn.GrnError):
+        s.set_screen_policy(7)
+    sur.close()
+    ctx.close()
