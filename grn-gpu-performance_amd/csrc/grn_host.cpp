@@ -101,6 +101,12 @@ struct TuneKey {
   }
 };
 
+// The device buffers of a batch: the context's own (grow-only, reused by
+// every grn_evaluate_batch) or a staged batch's.
+struct BatchBufs {
+  DevBuf progs, order, maxerr, out, batch, scratch;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -108,11 +114,10 @@ struct Ctx {
   int n_cu = 0;
   size_t max_lds = 0;
   std::vector<HostDataset> datasets;
-  DevBuf d_datasets, d_progs, d_order, d_maxerr, d_out, d_fields;
+  DevBuf d_datasets, d_fields;
+  BatchBufs bufs;  // the buffers of grn_evaluate_batch and its kin
   std::vector<const GrnKernelEntry *> kernels;
   std::map<const void *, size_t> lds_set;  // dynamic-LDS attribute already applied
-  DevBuf d_scratch;                         // halo-tiled state slots
-  DevBuf d_batch;                           // GrnBatch: per-launch work queues
   int tile_halo = 4;                        // Euler steps per tile load
   int launch_mode = GRN_LAUNCH_EAGER;
   std::map<std::vector<long>, hipGraphExec_t> graphs;  // by launch layout
@@ -270,21 +275,39 @@ struct Item {
   double max_err;
 };
 
-// Compiles, uploads, launches and collects one batch.  fields (optional)
-// receives each model's final field of experiment field_exp.
 // Options of a batch beyond the public arguments (autotuner timing runs).
 struct BatchOptions {
   const KernelChoice *force = nullptr;  // run every model with this configuration
   int step_cap = 0;                     // > 0: at most this many Euler steps
 };
 
-int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
-              const double *max_errors, const double *dt_override, int path, grn_result *out,
-              double *fields_host, long field_stride, int field_exp, int *n_products_out,
-              const BatchOptions &opt = BatchOptions()) {
-  const auto t_start = std::chrono::steady_clock::now();
-  if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
-  std::vector<Item> items(n);
+// A compiled and planned batch: everything its launches need but the device
+// buffers.  Queue position j runs model idx[j]; group g is queue g.
+struct Plan {
+  int n = 0;
+  std::vector<Item> items;  // input order
+  std::vector<int> idx;
+  std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
+  struct Launch {
+    KernelChoice kc;
+    int grid;
+  };
+  std::vector<Launch> launches;
+  GrnBatch batch{};
+  size_t scratch_bytes = 0;
+  int step_cap = 0;
+};
+
+// Compiles every model, chooses its kernel configuration, groups the models
+// by configuration (and grid, for tiled kernels), orders each group
+// costliest-first and sizes each group's persistent launch.
+int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, const double *max_errors,
+              const double *dt_override, int path, const BatchOptions &opt, Plan *P) {
+  if (n < 0 || (n > 0 && !models)) return fail(GRN_E_INVALID, "invalid batch arguments");
+  P->n = n;
+  P->step_cap = opt.step_cap;
+  std::vector<Item> &items = P->items;
+  items.assign(n, Item());
   for (int i = 0; i < n; ++i) {
     const int ds = dataset_ids ? dataset_ids[i] : 0;
     if (ds < 0 || ds >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "model %d: unknown dataset %d", i, ds);
@@ -335,7 +358,8 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
 
   // Group by kernel, costliest first inside each group (longest-processing-
   // time order for the persistent work queue).
-  std::vector<int> idx(n);
+  std::vector<int> &idx = P->idx;
+  idx.resize(n);
   for (int i = 0; i < n; ++i) idx[i] = i;
   auto same_class = [&](int a, int b) {
     const KernelChoice &x = items[a].kc, &y = items[b].kc;
@@ -358,26 +382,9 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     return items[a].cost > items[b].cost;
   });
 
-  std::vector<GrnProgram> progs(n);
-  std::vector<double> maxerr(n);
-  for (int i = 0; i < n; ++i) {
-    progs[i] = items[i].prog;
-    maxerr[i] = items[i].max_err;
-  }
-  int rc;
-  if ((rc = c->d_progs.reserve(sizeof(GrnProgram) * std::max(n, 1)))) return rc;
-  if ((rc = c->d_order.reserve(sizeof(int) * std::max(n, 1)))) return rc;
-  if ((rc = c->d_maxerr.reserve(sizeof(double) * std::max(n, 1)))) return rc;
-  if ((rc = c->d_out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
-  if ((rc = c->d_batch.reserve(sizeof(GrnBatch)))) return rc;
-  if (fields_host) {
-    if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
-    for (int i = 1; i < n; ++i)
-      if (!same_class(i, 0)) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
-  }
-
   // One persistent launch per kernel class.
-  std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
+  std::vector<std::pair<int, int>> &groups = P->groups;
+  groups.clear();
   for (int i = 0; i < n;) {
     int j = i;
     while (j < n && same_class(idx[j], idx[i])) ++j;
@@ -385,18 +392,14 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     i = j;
   }
   if ((int)groups.size() > GRN_MAX_QUEUES) return fail(GRN_E_CAPACITY, "too many kernel classes in one batch");
-  GrnBatch batch{};
-  batch.n_total = n;
-  batch.n_queues = (int)groups.size();
-  struct Launch {
-    KernelChoice kc;
-    int grid;
-    GrnLaunch L;
-  };
-  std::vector<Launch> plan;
+  P->batch = GrnBatch{};
+  P->batch.n_total = n;
+  P->batch.n_queues = (int)groups.size();
+  P->launches.clear();
+  P->scratch_bytes = 0;
   for (size_t gi = 0; gi < groups.size(); ++gi) {
     const int b = groups[gi].first, e = groups[gi].second;
-    batch.q[gi] = GrnQueue{0, e - b, b, 0};
+    P->batch.q[gi] = GrnQueue{0, e - b, b, 0};
     // the class's block size and LDS: the largest over its members
     KernelChoice kc = items[idx[b]].kc;
     for (int j = b; j < e; ++j) {
@@ -413,60 +416,86 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
     if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
     const int grid = std::min(e - b, per_cu * c->n_cu);
-    if (kc.slot_floats > 0 && (rc = c->d_scratch.reserve(sizeof(float) * (size_t)kc.slot_floats * grid))) return rc;
-    plan.push_back({kc, grid, GrnLaunch{}});
+    P->scratch_bytes = std::max(P->scratch_bytes, sizeof(float) * (size_t)kc.slot_floats * grid);
+    P->launches.push_back({kc, grid});
   }
-  // Every buffer is sized now: the launch arguments below stay valid.
-  for (size_t gi = 0; gi < plan.size(); ++gi) {
-    const KernelChoice &kc = plan[gi].kc;
-    GrnLaunch &L = plan[gi].L;
-    L.scratch = (float *)c->d_scratch.p;
-    L.slot_floats = kc.slot_floats;
-    L.tile_w = kc.tile.tw;
-    L.tile_h = kc.tile.th;
-    L.tile_sx = kc.tile.sx;
-    L.tile_sy = kc.tile.sy;
-    L.tile_halo = kc.tile.halo;
-    L.progs = (const GrnProgram *)c->d_progs.p;
-    L.queue = &((GrnBatch *)c->d_batch.p)->q[gi];
-    L.datasets = (const GrnDataset *)c->d_datasets.p;
-    L.max_err = (const double *)c->d_maxerr.p;
-    L.out = (GrnModelOut *)c->d_out.p;
-    L.fields = fields_host ? (double *)c->d_fields.p : nullptr;
-    L.field_stride = field_stride;
-    L.field_exp = field_exp;
-    L.step_cap = opt.step_cap;
-  }
+  return 0;
+}
 
+// Sizes the buffers and enqueues the uploads of programs, order, maxError
+// bounds and the batch table.  The host arrays are the plan's own.
+int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<GrnProgram> *progs, std::vector<double> *maxerr) {
+  const int n = P.n;
+  int rc;
+  if ((rc = B.progs.reserve(sizeof(GrnProgram) * std::max(n, 1)))) return rc;
+  if ((rc = B.order.reserve(sizeof(int) * std::max(n, 1)))) return rc;
+  if ((rc = B.maxerr.reserve(sizeof(double) * std::max(n, 1)))) return rc;
+  if ((rc = B.out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
+  if ((rc = B.batch.reserve(sizeof(GrnBatch)))) return rc;
+  if (P.scratch_bytes && (rc = B.scratch.reserve(P.scratch_bytes))) return rc;
+  progs->resize(n);
+  maxerr->resize(n);
+  for (int i = 0; i < n; ++i) {
+    (*progs)[i] = P.items[i].prog;
+    (*maxerr)[i] = P.items[i].max_err;
+  }
   hipStream_t st = c->stream;
   if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(c->d_progs.p, progs.data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(c->d_order.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(c->d_maxerr.p, maxerr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(B.progs.p, progs->data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(B.order.p, P.idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(B.maxerr.p, maxerr->data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   }
-  HIP_TRY(hipMemcpyAsync(c->d_batch.p, &batch, sizeof(GrnBatch), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(B.batch.p, &P.batch, sizeof(GrnBatch), hipMemcpyHostToDevice, st));
+  return 0;
+}
 
-  // The launches: result clearing, then one persistent kernel per class.
+// Enqueues the batch: result clearing (which also rewinds the work queues),
+// then one persistent kernel per class, eagerly or as a captured graph,
+// bracketed by the context's timing events.
+int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_stride, int field_exp,
+                bool *replayed) {
+  *replayed = false;
+  if (P.n == 0) return 0;
+  std::vector<GrnLaunch> L(P.launches.size());
+  for (size_t gi = 0; gi < P.launches.size(); ++gi) {
+    const KernelChoice &kc = P.launches[gi].kc;
+    GrnLaunch &l = L[gi];
+    l.scratch = (float *)B.scratch.p;
+    l.slot_floats = kc.slot_floats;
+    l.tile_w = kc.tile.tw;
+    l.tile_h = kc.tile.th;
+    l.tile_sx = kc.tile.sx;
+    l.tile_sy = kc.tile.sy;
+    l.tile_halo = kc.tile.halo;
+    l.progs = (const GrnProgram *)B.progs.p;
+    l.queue = &((GrnBatch *)B.batch.p)->q[gi];
+    l.datasets = (const GrnDataset *)c->d_datasets.p;
+    l.max_err = (const double *)B.maxerr.p;
+    l.out = (GrnModelOut *)B.out.p;
+    l.fields = fields ? (double *)fields->p : nullptr;
+    l.field_stride = field_stride;
+    l.field_exp = field_exp;
+    l.step_cap = P.step_cap;
+  }
+  hipStream_t st = c->stream;
   auto issue = [&]() -> int {
-    HIP_TRY(grn_launch_clear_results((const GrnBatch *)c->d_batch.p, (GrnModelOut *)c->d_out.p, st));
-    for (Launch &x : plan) {
-      const GrnProgram *pp = x.L.progs;
-      const GrnDataset *dp = x.L.datasets;
-      const int *op = (const int *)c->d_order.p;
-      void *args[] = {&x.L, &pp, &dp, &op};
+    HIP_TRY(grn_launch_clear_results((GrnBatch *)B.batch.p, (GrnModelOut *)B.out.p, st));
+    for (size_t gi = 0; gi < P.launches.size(); ++gi) {
+      const Plan::Launch &x = P.launches[gi];
+      const GrnProgram *pp = L[gi].progs;
+      const GrnDataset *dp = L[gi].datasets;
+      const int *op = (const int *)B.order.p;
+      void *args[] = {&L[gi], &pp, &dp, &op};
       HIP_TRY(hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, st));
     }
     return 0;
   };
-  bool replayed = false;
-  auto launch = [&]() -> int {
-  if (c->launch_mode == GRN_LAUNCH_GRAPH && n > 0) {
+  if (c->launch_mode == GRN_LAUNCH_GRAPH) {
     // the layout: everything a captured launch bakes in
-    std::vector<long> key = {(long)c->d_batch.p, (long)c->d_out.p, (long)c->d_progs.p, (long)c->d_order.p,
-                             (long)c->d_maxerr.p, (long)c->d_datasets.p, (long)c->d_scratch.p,
-                             (long)(fields_host ? c->d_fields.p : nullptr), field_stride, field_exp,
-                             opt.step_cap};
-    for (const Launch &x : plan) {
+    std::vector<long> key = {(long)B.batch.p, (long)B.out.p, (long)B.progs.p, (long)B.order.p,
+                             (long)B.maxerr.p, (long)c->d_datasets.p, (long)B.scratch.p,
+                             (long)(fields ? fields->p : nullptr), field_stride, field_exp, P.step_cap};
+    for (const Plan::Launch &x : P.launches) {
       key.insert(key.end(), {(long)x.kc.k->fn, x.grid, x.kc.block, (long)x.kc.lds, x.kc.slot_floats,
                              x.kc.tile.tw, x.kc.tile.th, x.kc.tile.sx, x.kc.tile.sy, x.kc.tile.halo});
     }
@@ -488,7 +517,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
       if (e_inst != hipSuccess) return fail(GRN_E_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e_inst));
       it = c->graphs.emplace(key, ge).first;
     } else {
-      replayed = true;
+      *replayed = true;
     }
     HIP_TRY(hipEventRecord(c->ev0, st));
     HIP_TRY(hipGraphLaunch(it->second, st));
@@ -500,37 +529,24 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     HIP_TRY(hipEventRecord(c->ev1, st));
   }
   return 0;
-  };
-  if ((rc = launch())) {
-    (void)hipStreamSynchronize(st);  // the uploads read host buffers of this frame
-    return rc;
-  }
-  const int launches = (int)plan.size();
+}
 
+// Waits for the batch, turns the kernels' outputs into results
+// (CEvaluatorMorphogens::evaluate, cevaluatormorphogens.cpp:239-242) and
+// records the context's statistics.
+int collect_plan(Ctx *c, const Plan &P, BatchBufs &B, bool replayed, grn_result *out, int *n_products_out,
+                 std::chrono::steady_clock::time_point t_start) {
+  const int n = P.n;
+  hipStream_t st = c->stream;
   std::vector<GrnModelOut> res(n);
-  if (n > 0)
-    HIP_TRY(hipMemcpyAsync(res.data(), c->d_out.p, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
-  if (fields_host && n > 0) {
-    // d_fields rows are indexed by queue position; return them in input order.
-    std::vector<double> tmp((size_t)field_stride * n);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), c->d_fields.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    // The kernel writes a model's field at its queue position inside its
-    // launch; map the rows back to input order.
-    for (size_t gi = 0; gi < groups.size(); ++gi)
-      for (int j = groups[gi].first; j < groups[gi].second; ++j) {
-        const int row = j - groups[gi].first;
-        std::memcpy(fields_host + (size_t)field_stride * idx[j], tmp.data() + (size_t)field_stride * row,
-                    sizeof(double) * field_stride);
-      }
-  }
+  if (n > 0) HIP_TRY(hipMemcpyAsync(res.data(), B.out.p, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-
+  if (n > 0) HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   double cell_steps = 0, flops = 0;
   for (int i = 0; i < n; ++i) {
-    const HostDataset &D = c->datasets[items[i].dataset];
+    const Item &it = P.items[i];
+    const HostDataset &D = c->datasets[it.dataset];
     const GrnModelOut &o = res[i];
     if (!o.done) return fail(GRN_E_RESULT, "model %d: the kernel wrote no result", i);
     grn_result &r = out[i];
@@ -538,10 +554,9 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     r.change = o.change;
     r.steps = o.steps;
     r.experiments = o.exps;
-    r.n_products = items[i].prog.G;
+    r.n_products = it.prog.G;
     r.reserved = 0;
-    // CEvaluatorMorphogens::evaluate (cevaluatormorphogens.cpp:239-242)
-    if (o.err > items[i].max_err)
+    if (o.err > it.max_err)
       r.fitness = 100;
     else if (D.params.error_precision > 0)
       r.fitness = grn_ceil_s(o.err, D.params.error_precision);
@@ -549,20 +564,71 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
       r.fitness = o.err;
     const double cs = (double)D.params.dim_x * D.params.dim_y * o.steps;
     cell_steps += cs;
-    flops += cs * items[i].info.flops_per_cell_step;
-    if (n_products_out) n_products_out[i] = items[i].prog.G;
+    flops += cs * it.info.flops_per_cell_step;
+    if (n_products_out) n_products_out[i] = it.prog.G;
   }
   c->stats.kernel_ms = ms;
   c->stats.cell_steps = cell_steps;
   c->stats.alg_flops = flops;
-  c->stats.n_launches = launches;
+  c->stats.n_launches = (int)P.launches.size();
   c->stats.n_models = n;
   c->stats.launch_mode = c->launch_mode;
   c->stats.graph_replayed = replayed ? 1 : 0;
-  c->stats.host_ms =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  c->stats.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   return 0;
 }
+
+// Compiles, uploads, launches and collects one batch on the context's own
+// buffers.  fields (optional) receives each model's final field of
+// experiment field_exp.
+int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
+              const double *max_errors, const double *dt_override, int path, grn_result *out,
+              double *fields_host, long field_stride, int field_exp, int *n_products_out,
+              const BatchOptions &opt = BatchOptions()) {
+  const auto t_start = std::chrono::steady_clock::now();
+  if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
+  Plan P;
+  int rc = make_plan(c, n, dataset_ids, models, max_errors, dt_override, path, opt, &P);
+  if (rc) return rc;
+  if (fields_host) {
+    if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
+    if (P.groups.size() > 1) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
+  }
+  std::vector<GrnProgram> progs;
+  std::vector<double> maxerr;
+  bool replayed = false;
+  if ((rc = upload_plan(c, P, c->bufs, &progs, &maxerr)) ||
+      (rc = launch_plan(c, P, c->bufs, fields_host ? &c->d_fields : nullptr, field_stride, field_exp, &replayed))) {
+    (void)hipStreamSynchronize(c->stream);  // the uploads read host buffers of this frame
+    return rc;
+  }
+  if (fields_host && n > 0) {
+    // d_fields rows are indexed by queue position; return them in input order.
+    std::vector<double> tmp((size_t)field_stride * n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), c->d_fields.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    // The kernel writes a model's field at its queue position inside its
+    // launch; map the rows back to input order.
+    for (size_t gi = 0; gi < P.groups.size(); ++gi)
+      for (int j = P.groups[gi].first; j < P.groups[gi].second; ++j) {
+        const int row = j - P.groups[gi].first;
+        std::memcpy(fields_host + (size_t)field_stride * P.idx[j], tmp.data() + (size_t)field_stride * row,
+                    sizeof(double) * field_stride);
+      }
+  }
+  return collect_plan(c, P, c->bufs, replayed, out, n_products_out, t_start);
+}
+
+// A batch compiled, planned and uploaded once (grn_stage_batch), launched by
+// grn_run_staged with nothing crossing the bus but the results.
+struct Staged {
+  Ctx *c = nullptr;
+  Plan P;
+  BatchBufs B;
+  std::vector<GrnProgram> progs;  // sources of the uploads
+  std::vector<double> maxerr;
+};
 
 }  // namespace
 
@@ -699,6 +765,49 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids, const gr
   if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
   HIP_TRY(hipSetDevice(c->device));
   return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, path, out, nullptr, 0, 0, nullptr);
+}
+
+int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
+                    const double *max_errors, const double *dt_override, int path, void **staged) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !staged) return fail(GRN_E_INVALID, "null argument");
+  *staged = nullptr;
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
+  if (n_models < 1 || !models) return fail(GRN_E_INVALID, "a staged batch needs at least one model");
+  HIP_TRY(hipSetDevice(c->device));
+  std::unique_ptr<Staged> S(new Staged);
+  S->c = c;
+  int rc = make_plan(c, n_models, dataset_ids, models, max_errors, dt_override, path, BatchOptions(), &S->P);
+  if (!rc) rc = upload_plan(c, S->P, S->B, &S->progs, &S->maxerr);
+  const hipError_t e = hipStreamSynchronize(c->stream);  // the uploads read S's host arrays
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+  *staged = S.release();
+  return 0;
+}
+
+int grn_run_staged(void *ctx, void *staged, grn_result *out) {
+  Ctx *c = as_ctx(ctx);
+  Staged *S = static_cast<Staged *>(staged);
+  if (!c || !S || !out) return fail(GRN_E_INVALID, "null argument");
+  if (S->c != c) return fail(GRN_E_INVALID, "the batch was staged on another context");
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t_start = std::chrono::steady_clock::now();
+  bool replayed = false;
+  int rc = launch_plan(c, S->P, S->B, nullptr, 0, 0, &replayed);
+  if (rc) {
+    (void)hipStreamSynchronize(c->stream);
+    return rc;
+  }
+  return collect_plan(c, S->P, S->B, replayed, out, nullptr, t_start);
+}
+
+void grn_staged_destroy(void *staged) {
+  Staged *S = static_cast<Staged *>(staged);
+  if (!S) return;
+  (void)hipSetDevice(S->c->device);
+  (void)hipStreamSynchronize(S->c->stream);
+  delete S;
 }
 
 int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, double dt_override, int experiment,

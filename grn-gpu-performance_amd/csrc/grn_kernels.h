@@ -25,6 +25,6 @@ extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
 
 // Zeroes out[0 .. batch->n_total) on stream (grn_probe.hip).
-hipError_t grn_launch_clear_results(const GrnBatch *batch, GrnModelOut *out, hipStream_t stream);
+hipError_t grn_launch_clear_results(GrnBatch *batch, GrnModelOut *out, hipStream_t stream);
 
 #endif

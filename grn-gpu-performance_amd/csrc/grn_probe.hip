@@ -14,9 +14,11 @@
 #include "grn_kernels.h"
 #include "grn_program.h"
 
-__global__ __launch_bounds__(256) void grn_clear_results_kernel(const GrnBatch *__restrict__ batch,
+__global__ __launch_bounds__(256) void grn_clear_results_kernel(GrnBatch *__restrict__ batch,
                                                                 GrnModelOut *__restrict__ out) {
   const int n = batch->n_total;
+  // rewind the work queues: the same batch table can be launched again
+  if (blockIdx.x == 0 && (int)threadIdx.x < batch->n_queues) batch->q[threadIdx.x].head = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     GrnModelOut z;
     z.err = 0.0;
@@ -29,7 +31,7 @@ __global__ __launch_bounds__(256) void grn_clear_results_kernel(const GrnBatch *
   }
 }
 
-hipError_t grn_launch_clear_results(const GrnBatch *batch, GrnModelOut *out, hipStream_t stream) {
+hipError_t grn_launch_clear_results(GrnBatch *batch, GrnModelOut *out, hipStream_t stream) {
   hipLaunchKernelGGL(grn_clear_results_kernel, dim3(64), dim3(256), 0, stream, batch, out);
   return hipGetLastError();
 }

@@ -38,7 +38,8 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
-    "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_simulate_fields",
+    "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_stage_batch", "grn_run_staged",
+    "grn_staged_destroy", "grn_simulate_fields",
     "grn_get_stats", "grn_test_performance", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
     "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
@@ -203,6 +204,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
                                  ctypes.POINTER(_Result)]
     lib.grn_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp]
+    lib.grn_stage_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.grn_run_staged.argtypes = [vp, vp, vp]
+    lib.grn_staged_destroy.argtypes = [vp]
+    lib.grn_staged_destroy.restype = None
     lib.grn_simulate_fields.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int, vp, ctypes.c_long, ip]
     lib.grn_get_stats.argtypes = [vp, ctypes.POINTER(_Stats)]
@@ -348,6 +353,30 @@ class SimParams:
                           self.dist_error_threshold, self.error_precision, self.fitness_kind)
 
 
+class StagedBatch:
+    """A batch resident on the device (Context.stage_batch)."""
+
+    def __init__(self, ctx, handle, n):
+        self._ctx, self._h, self.n = ctx, handle, n
+        self._lib = ctx._lib
+
+    def run(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), RESULT_DTYPE)
+        _check(self._lib.grn_run_staged(self._ctx._h, self._h, out.ctypes.data))
+        return out[: self.n]
+
+    def close(self):
+        if self._h:
+            self._lib.grn_staged_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class ModelBatch:
     """Models marshalled once into the C array grn_evaluate_batch reads (the
     way a C++ caller keeps its grn_model array); the models must not change
@@ -430,6 +459,26 @@ class Context:
             None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data,
             path, out.ctypes.data))
         return out[:n]
+
+    def stage_batch(self, models, datasets=None, max_errors=None, dt=None,
+                    path: int = PATH_AUTO) -> "StagedBatch":
+        """Compiles and uploads a batch once; StagedBatch.run() evaluates it
+        with the programs resident on the device (same results as
+        evaluate_batch on the same arguments)."""
+        n = len(models)
+        cms = models.array if isinstance(models, ModelBatch) else (_Model * max(n, 1))(*[m._c() for m in models])
+        ds = None if datasets is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
+        me = None if max_errors is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(max_errors, np.float64), (n,)))
+        dts = None if dt is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+        h = ctypes.c_void_p()
+        _check(self._lib.grn_stage_batch(
+            self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
+            None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data,
+            path, ctypes.byref(h)))
+        return StagedBatch(self, h.value, n)
 
     def simulate_fields(self, model: Model, dataset: int = 0, experiment: int = 0,
                         dt: float = 0.0, path: int = PATH_AUTO) -> np.ndarray:

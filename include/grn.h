@@ -167,6 +167,18 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
                        const grn_model *models, const double *max_errors,
                        const double *dt_override, int path, grn_result *out);
 
+/* A batch compiled, planned and uploaded once: grn_run_staged evaluates it
+ * again and again with the programs resident in device memory (only the
+ * results cross the bus), giving the same results as grn_evaluate_batch on
+ * the same arguments.  For repeated evaluation of one population (timing,
+ * the known-answer self-test); no counterpart in the reference, which
+ * uploads every batch.  The handle belongs to ctx; grn_staged_destroy frees
+ * it (before grn_destroy of its context). */
+int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
+                    const double *max_errors, const double *dt_override, int path, void **staged);
+int grn_run_staged(void *ctx, void *staged, grn_result *out);
+void grn_staged_destroy(void *staged);
+
 /* Simulates experiment `experiment` of a dataset for one model and writes
  * the final field, fields[n_products][dim_y][dim_x] as double, capacity in
  * elements.  Runs the same kernel code as grn_evaluate. */

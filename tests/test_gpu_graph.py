@@ -86,3 +86,29 @@ def test_search_through_captured_generations_equals_eager(ctx):
         assert [m.to_text(17) for m in pa[0]] == [m.to_text(17) for m in pb[0]]
     assert a.best()[1] == b.best()[1]
     print(f"[gpu] graph replays: {replays}", flush=True)
+
+
+def test_staged_batch_equals_evaluate_batch(ctx):
+    """grn_stage_batch / grn_run_staged: a batch uploaded once and launched
+    repeatedly (the clear kernel rewinds every class's work queue) returns
+    exactly what grn_evaluate_batch returns, eagerly and as a replayed graph,
+    with other batches and a new dataset upload in between."""
+    ids, models = mixed_batch(ctx, seed=3)
+    ids2, models2 = mixed_batch(ctx, seed=4)
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    want = ctx.evaluate_batch(models, datasets=ids)
+    want2 = ctx.evaluate_batch(models2, datasets=ids2)
+    staged = ctx.stage_batch(models, datasets=ids)
+    for mode in (grn.LAUNCH_EAGER, grn.LAUNCH_GRAPH):
+        for rep in range(3):
+            print(f"[gpu] staged run, mode {mode}, repeat {rep}", flush=True)
+            ctx.set_launch_mode(mode)
+            got = staged.run()
+            assert np.array_equal(got, want)
+            assert ctx.stats()["n_launches"] >= 3
+            ctx.set_launch_mode(grn.LAUNCH_EAGER)
+            assert np.array_equal(ctx.evaluate_batch(models2, datasets=ids2), want2)
+    mixed_batch(ctx, seed=5)  # more datasets: the device dataset table may move
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    assert np.array_equal(staged.run(), want)
+    staged.close()
