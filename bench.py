@@ -43,7 +43,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, budget_s=25.0):
+def cpu_baseline(w, budget_s=22.0):
     """FP64 oracle on the host cores over a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import grn_oracle as ora
@@ -63,9 +63,22 @@ def cpu_baseline(w, budget_s=25.0):
     r = ora.evaluate_batch(p, w.init, w.target, sample, n_threads=threads)
     dt = time.perf_counter() - t0
     cs = float(np.sum(r["steps"]) * cells)
+    # one thread on three more models
+    one = w.models[len(probe) + n:len(probe) + n + 3]
+    t1 = time.perf_counter()
+    r1 = ora.evaluate_batch(p, w.init, w.target, one, n_threads=1)
+    dt1 = time.perf_counter() - t1
+    cpu_model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": cs / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
             "sample": f"{len(sample)} of the {len(w.models)} models at full 2000 steps, FP64, "
-                      f"{threads} threads, {dt:.1f} s"}
+                      f"{threads} threads, {dt:.1f} s",
+            "single_thread_value": float(np.sum(r1["steps"]) * cells) / dt1,
+            "single_thread_sample": f"{len(one)} models, {dt1:.1f} s", "cpu_model": cpu_model}
 
 
 def bench_islands(args):
