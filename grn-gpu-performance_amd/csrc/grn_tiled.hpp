@@ -72,6 +72,13 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   const double max_err = a.max_err[pi];
   const int plane_floats = G * f.plane;
 
+  // The LDS window planes start at zero once per model: the one-cell ring
+  // around the window (the no-flux zeros) is never written, the window cells
+  // of diffusing genes are rewritten by every tile before they are read, and
+  // the planes of genes that do not diffuse stay zero (their diffusion term
+  // is multiplied by dif = 0, so they must hold finite values).
+  for (int i = tid; i < plane_floats; i += blockDim.x) sh.lds[i] = 0.f;
+
   double err = 0.0, chg_acc = 0.0;
   int e = 0, steps_total = 0;
   while (e < n_exp && err <= max_err) {
@@ -117,8 +124,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
               f.hi[g][k] = gidx[k] >= 0 ? vh : 0.f;
               f.lo[g][k] = gidx[k] >= 0 ? vl : 0.f;
             }
-          for (int i = tid; i < plane_floats; i += blockDim.x) sh.lds[i] = 0.f;
-          __syncthreads();
+          __syncthreads();  // every read of the previous tile's planes is done
           f.store_planes();
           __syncthreads();
           // (the general step only: a second step variant in these larger
