@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import grn
+import evolve
 import grn_oracle as ora
 import workloads
 
@@ -225,6 +226,42 @@ def test_tiled_path_parity(ctx, W, H, gr):
     _, fo = ora.evaluate(ora.from_simparams(p), init, target, models[0].genes, models[0].links,
                          want_fields=True)
     check_fields(f, fo)
+
+
+def gate_variants(rng, n):
+    """The hidden 10-gene blob/gradient gate network and n - 1 copies with
+    every K and n scaled by factors in [0.7, 1.3]: fields with structure
+    that persists to the end of a long run."""
+    base = evolve.hidden_blob_gate()
+    out = [base]
+    for _ in range(n - 1):
+        links = base.links.copy()
+        links["dis_const"] *= rng.uniform(0.7, 1.3, len(links))
+        links["hill_coef"] *= rng.uniform(0.7, 1.3, len(links))
+        out.append(grn.Model(base.genes.copy(), links))
+    return out
+
+
+@pytest.mark.parametrize("side", [128, 96])
+def test_tiled_path_parity_full_length(ctx, side):
+    """Config-2 sides at the configuration's full 3000 steps, dt 1 (the
+    time-step rule's cap): the compensated FP32 state of the halo-tiled path
+    stays within the fitness and field tolerances of the FP64 oracle over the
+    whole run."""
+    rng = np.random.default_rng(side)
+    prob = evolve.scaled_config4(seed=side, side=side, steps=3000)
+    p, init = prob.params, prob.init
+    target = rng.uniform(0, 1, (1, 1, side, side))
+    models = gate_variants(rng, 4)
+    ds, got, want = run_both(ctx, p, init, target, models, n_threads=4)
+    assert np.all(got["steps"] == 3000)
+    check_fitness(got["raw_error"], want["raw_error"])
+    # every gene's final field, cell by cell, for two of the networks
+    for m in models[:2]:
+        f = ctx.simulate_fields(m, dataset=ds)
+        _, fo = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, want_fields=True)
+        assert np.abs(fo[4]).max() > 0.1  # the output gene is on somewhere
+        check_fields(f, fo)
 
 
 @pytest.mark.parametrize("W,H", [(64, 64), (48, 48), (32, 32)])
