@@ -124,9 +124,105 @@ struct Model {
     return best;
   }
 
+  // R = lim*h - deg*c (+ dif * the 5-point Laplacian from the LDS mirror
+  // when DIF) and the compensated Euler update of gene p for the cells
+  // k0 .. k0 + CB - 1.
+  template <bool TRACK, bool FAST, bool DIF, int CB>
+  __device__ __forceinline__ void update_cells_n(int p, const GrnGene &gp, const float *hv, int k0, float *cmax) {
+    const float *pl = sh.lds + p * plane + ubase;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int k = k0 + j;
+      const float c = hi[p][k];
+      const float h = hv[j];
+      float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
+      if (DIF) {
+        const float left = pl[(k + 1) * pc()];
+        const float right = pl[(k + 1) * pc() + 2];
+        const float up = pl[k * pc() + 1];
+        const float down = pl[(k + 2) * pc() + 1];
+        const float S = ((left + right) + up) + down;
+        float lap = __builtin_fmaf(-nf[k], c, S);
+        if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
+          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+          lap = use ? lap : 0.f;
+        }
+        R = __builtin_fmaf(gp.dif, lap, R);
+      }
+      // Compensated Euler update c += dt*R (Fast2Sum on the pair) with
+      // the minConc clamp of modelsimulator.cpp:263-275.
+      const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
+      const float sv = c + yv;
+      const float nl = yv - (sv - c);
+      if (FAST) {
+        hi[p][k] = sv;
+        lo[p][k] = nl;
+      } else {
+        const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));  // branch-free
+        hi[p][k] = clamp ? 0.f : sv;
+        lo[p][k] = clamp ? 0.f : nl;
+      }
+      if (TRACK) *cmax = __builtin_fmaxf(*cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
+    }
+  }
+  template <bool TRACK, bool FAST, bool DIF>
+  __device__ __forceinline__ void update_cells(int p, const GrnGene &gp, const float *hv, int k0, float *cmax) {
+    update_cells_n<TRACK, FAST, DIF, cell_block()>(p, gp, hv, k0, cmax);
+  }
+  // The same with the diffusion term only where dif_on (uniform): the four
+  // LDS reads and the Laplacian are skipped for a gene that does not diffuse.
+  template <bool TRACK, bool FAST>
+  __device__ __forceinline__ void update_cells_masked(int p, const GrnGene &gp, const float *hv, int k0, int dif_on,
+                                                      float *cmax) {
+    constexpr int CB = cell_block();
+    float Rv[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) Rv[j] = __builtin_fmaf(gp.lim, hv[j], -gp.deg * hi[p][k0 + j]);
+    if (dif_on) {
+      const float *pl = sh.lds + p * plane + ubase;
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        const int k = k0 + j;
+        const float c = hi[p][k];
+        const float left = pl[(k + 1) * pc()];
+        const float right = pl[(k + 1) * pc() + 2];
+        const float up = pl[k * pc() + 1];
+        const float down = pl[(k + 2) * pc() + 1];
+        const float S = ((left + right) + up) + down;
+        float lap = __builtin_fmaf(-nf[k], c, S);
+        if (GUARD) {
+          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+          lap = use ? lap : 0.f;
+        }
+        Rv[j] = __builtin_fmaf(gp.dif, lap, Rv[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      const int k = k0 + j;
+      const float c = hi[p][k];
+      const float R = Rv[j];
+      const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
+      const float sv = c + yv;
+      const float nl = yv - (sv - c);
+      if (FAST) {
+        hi[p][k] = sv;
+        lo[p][k] = nl;
+      } else {
+        const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));
+        hi[p][k] = clamp ? 0.f : sv;
+        lo[p][k] = clamp ? 0.f : nl;
+      }
+      if (TRACK) *cmax = __builtin_fmaxf(*cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
+    }
+  }
+
   // FAST: the program guarantees a nonnegative state (no minConc clamp) and
   // no zero Hill exponent (no log2 floor), see GrnProgram::fast.
-  template <bool TRACK, bool FAST>
+  // MASKED: some genes do not diffuse (GrnProgram::dif_mask); their
+  // diffusion term is skipped by a uniform branch per gene.  Models whose
+  // genes all diffuse run the branch-free loop.
+  template <bool TRACK, bool FAST, bool MASKED = false>
   __device__ __forceinline__ float step() {
     typedef typename LgVec<G>::T lg_t;
     constexpr int CB = cell_block();
@@ -204,39 +300,10 @@ struct Model {
 #pragma unroll
           for (int j = 0; j < CB; ++j) hv[j] = 0.f;
         }
-        const float *pl = sh.lds + p * plane + ubase;
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-          const int k = k0 + j;
-          const float c = hi[p][k];
-          const float h = hv[j];
-          float R = __builtin_fmaf(gp.lim, h, -gp.deg * c);
-          const float left = pl[(k + 1) * pc()];
-          const float right = pl[(k + 1) * pc() + 2];
-          const float up = pl[k * pc() + 1];
-          const float down = pl[(k + 2) * pc() + 1];
-          const float S = ((left + right) + up) + down;
-          float lap = __builtin_fmaf(-nf[k], c, S);
-          if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
-            const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
-            lap = use ? lap : 0.f;
-          }
-          R = __builtin_fmaf(gp.dif, lap, R);
-          // Compensated Euler update c += dt*R (Fast2Sum on the pair) with
-          // the minConc clamp of modelsimulator.cpp:263-275.
-          const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
-          const float sv = c + yv;
-          const float nl = yv - (sv - c);
-          if (FAST) {
-            hi[p][k] = sv;
-            lo[p][k] = nl;
-          } else {
-            const bool clamp = (sv < min_conc) & (clamp_any_r | (R != 0.f));  // branch-free
-            hi[p][k] = clamp ? 0.f : sv;
-            lo[p][k] = clamp ? 0.f : nl;
-          }
-          if (TRACK) cmax = __builtin_fmaxf(cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
-        }
+        if constexpr (MASKED)
+          update_cells_masked<TRACK, FAST>(p, gp, hv, k0, (Pq->dif_mask >> p) & 1, &cmax);
+        else
+          update_cells<TRACK, FAST, true>(p, gp, hv, k0, &cmax);
         // Take this gene's results here so the compiler cannot sink its
         // update below the next genes' link loops (which would keep every
         // gene's constants and partial sums alive at once).
@@ -265,19 +332,19 @@ struct Model {
   // Simulator::simulate (simulator.cpp:54-63): run while step < S and the
   // last step's max change exceeds minConcChange.  A negative threshold can
   // never stop the loop, so then only the final step needs the reduction.
-  template <bool FAST>
+  template <bool FAST, bool MASKED>
   __device__ __forceinline__ float run_steps(int S, float mcc, int *steps_done) {
     float chg = 1.0f;
     int s = 0;
     if (mcc >= 0.f) {
       while (s < S && chg > mcc) {
-        chg = step<true, FAST>();
+        chg = step<true, FAST, MASKED>();
         ++s;
       }
     } else {
-      for (; s < S - 1; ++s) step<false, FAST>();
+      for (; s < S - 1; ++s) step<false, FAST, MASKED>();
       if (S > 0) {
-        chg = step<true, FAST>();
+        chg = step<true, FAST, MASKED>();
         ++s;
       }
     }
@@ -367,7 +434,15 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
     __syncthreads();
 
     int sdone = 0;
-    const float chg = P->fast ? f.template run_steps<true>(S, mcc, &sdone) : f.template run_steps<false>(S, mcc, &sdone);
+    // a model with non-diffusing genes skips their diffusion terms; a model
+    // whose genes all diffuse runs the branch-free loops (the any-width
+    // fallback kernels keep the branch-free loops only, to bound code size)
+    const bool masked = WC != 0 && P->dif_mask != (1 << G) - 1;
+    float chg;
+    if (P->fast)
+      chg = masked ? f.template run_steps<true, true>(S, mcc, &sdone) : f.template run_steps<true, false>(S, mcc, &sdone);
+    else
+      chg = masked ? f.template run_steps<false, true>(S, mcc, &sdone) : f.template run_steps<false, false>(S, mcc, &sdone);
     steps_total += sdone;
     chg_acc += (double)chg;
 

@@ -3,7 +3,6 @@
 
 static const GrnKernelEntry kTable[] = {
     GRN_FIELD_ENTRY(32, 2, 512, 2, 8, false),
-    GRN_FIELD_ENTRY(32, 4, 256, 2, 16, false),
 };
 
 extern "C" const GrnKernelEntry *grn_field_table_w32(int *n) {

@@ -1,12 +1,9 @@
-// On-chip field kernels for any grid width (runtime LDS pitch); the GUARD
-// instantiations serve fields that can turn negative.
+// On-chip field kernels for any grid width (runtime LDS pitch).
 #include "grn_field.hpp"
 
 static const GrnKernelEntry kTable[] = {
     GRN_FIELD_ENTRY(0, 4, 1024, 2, 8, false),
     GRN_FIELD_ENTRY(0, 1, 1024, 2, 16, false),
-    GRN_FIELD_ENTRY(0, 4, 1024, 2, 8, true),
-    GRN_FIELD_ENTRY(0, 1, 1024, 2, 16, true),
 };
 
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n) {

@@ -143,8 +143,9 @@ struct Ctx {
 };
 
 void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
-  const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w48,
-                                              grn_field_table_w64, grn_field_table_wany,
+  const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w32wide, grn_field_table_w32wide9,
+                                              grn_field_table_w32wide13, grn_field_table_w48,
+                                              grn_field_table_w64, grn_field_table_wany, grn_field_table_wanyguard,
                                               grn_field_table_tiled512, grn_field_table_tiled};
   for (auto fn : tables) {
     int n = 0;

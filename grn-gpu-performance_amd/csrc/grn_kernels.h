@@ -18,9 +18,13 @@ struct GrnKernelEntry {
 
 // Instantiations of the on-chip field kernel, one table per translation unit.
 extern "C" const GrnKernelEntry *grn_field_table_w32(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w32wide(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w32wide9(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w32wide13(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_wanyguard(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
 

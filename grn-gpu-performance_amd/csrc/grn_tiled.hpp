@@ -127,14 +127,14 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
           __syncthreads();  // every read of the previous tile's planes is done
           f.store_planes();
           __syncthreads();
-          // (the general step only: a second step variant in these larger
-          // kernels costs register spills that outweigh what it saves)
+          // (one step variant, with the per-gene diffusion branch: a second
+          // copy of the step loop in these kernels costs register spills)
           float c = 0.f;
           for (int t = 0; t < Tb; ++t) {
             if (track && t == Tb - 1)
-              c = f.template step<true, false>();
+              c = f.template step<true, false, true>();
             else
-              f.template step<false, false>();
+              f.template step<false, false, true>();
           }
           cblk = __builtin_fmaxf(cblk, c);
 #pragma unroll
