@@ -115,7 +115,7 @@ struct Model {
   static constexpr int lg_width() { return G <= 4 ? 4 : (G <= 8 ? 8 : 16); }
   static constexpr int budget() { return 131072 / NTMAX < 512 ? 131072 / NTMAX : 512; }
   static constexpr bool fits(int cb) {
-    return 2 * G * CPT + (TILED ? 3 * CPT : 0) + lg_width() * cb + 4 * cb + 32 <= budget();
+    return 2 * G * CPT + (TILED ? 3 * CPT + 32 : 24) + lg_width() * cb + 4 * cb <= budget();
   }
   static constexpr int cell_block() {
     int best = 1;
