@@ -36,7 +36,7 @@ FP32_PEAK_TFLOPS = 157.3
 # x 1.0 (scale factors from the copy-kernel calibration) as measured by
 # rocprofv3 --pmc; the commands and raw counts are in
 # profiles/field_kernel_config1_64x64.md.
-MEASURED_TRAFFIC_BYTES = 1398.0e6
+MEASURED_TRAFFIC_BYTES = 1545.3e6
 
 
 def log(msg):
