@@ -110,7 +110,9 @@ struct BatchBufs {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // a batch's odd kernel classes (default priority, like stream)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int n_cu = 0;
   size_t max_lds = 0;
   std::vector<HostDataset> datasets;
@@ -138,6 +140,9 @@ struct Ctx {
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -292,6 +297,7 @@ struct Plan {
   struct Launch {
     KernelChoice kc;
     int grid;
+    size_t scratch_off;  // floats: the launch's own state slots (launches may run concurrently)
   };
   std::vector<Launch> launches;
   GrnBatch batch{};
@@ -417,8 +423,9 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
     if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
     const int grid = std::min(e - b, per_cu * c->n_cu);
-    P->scratch_bytes = std::max(P->scratch_bytes, sizeof(float) * (size_t)kc.slot_floats * grid);
-    P->launches.push_back({kc, grid});
+    const size_t off = P->scratch_bytes / sizeof(float);
+    P->scratch_bytes += sizeof(float) * (size_t)kc.slot_floats * grid;
+    P->launches.push_back({kc, grid, off});
   }
   return 0;
 }
@@ -461,7 +468,7 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
   for (size_t gi = 0; gi < P.launches.size(); ++gi) {
     const KernelChoice &kc = P.launches[gi].kc;
     GrnLaunch &l = L[gi];
-    l.scratch = (float *)B.scratch.p;
+    l.scratch = (float *)B.scratch.p + P.launches[gi].scratch_off;
     l.slot_floats = kc.slot_floats;
     l.tile_w = kc.tile.tw;
     l.tile_h = kc.tile.th;
@@ -479,15 +486,28 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
     l.step_cap = P.step_cap;
   }
   hipStream_t st = c->stream;
+  // With several kernel classes the odd ones run on the second stream, so
+  // two persistent launches share the device and one class's drain overlaps
+  // the other's work (fork after the clearing kernel, join before the end
+  // event; the same pattern is captured into a graph).
+  const bool fork = P.launches.size() > 1;
   auto issue = [&]() -> int {
     HIP_TRY(grn_launch_clear_results((GrnBatch *)B.batch.p, (GrnModelOut *)B.out.p, st));
+    if (fork) {
+      HIP_TRY(hipEventRecord(c->ev_fork, st));
+      HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    }
     for (size_t gi = 0; gi < P.launches.size(); ++gi) {
       const Plan::Launch &x = P.launches[gi];
       const GrnProgram *pp = L[gi].progs;
       const GrnDataset *dp = L[gi].datasets;
       const int *op = (const int *)B.order.p;
       void *args[] = {&L[gi], &pp, &dp, &op};
-      HIP_TRY(hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, st));
+      HIP_TRY(hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, gi % 2 ? c->stream2 : st));
+    }
+    if (fork) {
+      HIP_TRY(hipEventRecord(c->ev_join, c->stream2));
+      HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
     return 0;
   };
@@ -678,8 +698,11 @@ int grn_create(int device, void **out) {
   std::unique_ptr<Ctx> c(new Ctx);
   c->device = device;
   HIP_TRY(hipStreamCreate(&c->stream));
+  HIP_TRY(hipStreamCreate(&c->stream2));
   HIP_TRY(hipEventCreate(&c->ev0));
   HIP_TRY(hipEventCreate(&c->ev1));
+  HIP_TRY(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
   int lds = 0;
   HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -695,6 +718,7 @@ void grn_destroy(void *ctx) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream2);
   delete c;
 }
 
