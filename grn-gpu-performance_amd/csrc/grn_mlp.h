@@ -38,6 +38,12 @@ struct MlpGemm {
   __bf16 *H;
   __bf16 *Ht;
   const float *Zprev;
+  // EPI_F32 only: ksplit > 1 slices K over that many workgroups per tile;
+  // each slice writes its partial product to Cpart[slice][M][N] and a second
+  // kernel sums the slices into C in slice order (deterministic).  K must be
+  // a multiple of 32 * ksplit.
+  int ksplit;
+  float *Cpart;
 };
 
 struct MlpAdam {

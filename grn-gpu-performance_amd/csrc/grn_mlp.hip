@@ -17,6 +17,8 @@
 //   D: rows 4*(l/16) + r (r = 0..3), column l%16
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "grn_mlp.h"
 
 namespace grn_mlp {
@@ -48,11 +50,14 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int K = g.K;
+  // this workgroup's K slice (the whole of K unless split)
+  const int ks = (EPI == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
+  const int kbeg = blockIdx.z * (K / ks), kend = kbeg + K / ks;
 
   // each thread stages 16 bf16 (two 16-byte vectors) of one A row and one Bt row
   const int lrow = t >> 1, lhalf = t & 1;
-  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + lrow) * K + 16 * lhalf);
-  const uint4 *pb = (const uint4 *)(g.Bt + (size_t)(n0 + lrow) * K + 16 * lhalf);
+  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + lrow) * K + kbeg + 16 * lhalf);
+  const uint4 *pb = (const uint4 *)(g.Bt + (size_t)(n0 + lrow) * K + kbeg + 16 * lhalf);
   uint4 ra0 = pa[0], ra1 = pa[1], rb0 = pb[0], rb1 = pb[1];
 
   f32x4 acc[4][4];
@@ -64,14 +69,14 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
   uint4 *sa = (uint4 *)(As + lrow * LDS_STRIDE + 16 * lhalf);
   uint4 *sb = (uint4 *)(Bs + lrow * LDS_STRIDE + 16 * lhalf);
   const int frow = lane & 15, fk = 8 * (lane >> 4);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();  // the previous tile's fragment reads are done
     sa[0] = ra0;
     sa[1] = ra1;
     sb[0] = rb0;
     sb[1] = rb1;
     __syncthreads();
-    if (k0 + BK < K) {  // prefetch the next K slice while this one multiplies
+    if (k0 + BK < kend) {  // prefetch the next K slice while this one multiplies
       pa += 4;
       pb += 4;
       ra0 = pa[0];
@@ -91,6 +96,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
   }
 
   const int M = g.M, N = g.N;
+  float *Cout = ks > 1 ? g.Cpart + (size_t)blockIdx.z * M * N : g.C;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
           x *= gelu_grad(g.Zprev[o]);
           g.H[o] = (bf16)x;
         } else {
-          g.C[o] = x;
+          Cout[o] = x;
         }
         v[r] = x;
       }
@@ -122,6 +128,23 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
         }
       }
     }
+}
+
+// C = the sum of the ks partial products in slice order, four floats per
+// thread.
+__global__ __launch_bounds__(256) void ksum_kernel(const float4 *__restrict__ part, long n4, int ks,
+                                                   float4 *__restrict__ C) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+    float4 s = part[i];
+    for (int z = 1; z < ks; ++z) {
+      const float4 v = part[(size_t)z * n4 + i];
+      s.x += v.x;
+      s.y += v.y;
+      s.z += v.z;
+      s.w += v.w;
+    }
+    C[i] = s;
+  }
 }
 
 // Output layer, one wave per row: y[b] = H[b] . w + b0 (512 inputs).
@@ -238,13 +261,20 @@ __global__ __launch_bounds__(256) void gather_kernel(const bf16 *__restrict__ fe
 using namespace grn_mlp;
 
 hipError_t grn_mlp_gemm(int epi, const MlpGemm &g, hipStream_t s) {
-  const dim3 grid(g.N / BN, g.M / BM), block(256);
+  const int ks = (epi == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
+  if (ks > 1 && (!g.Cpart || g.K % (BK * ks))) return hipErrorInvalidValue;
+  const dim3 grid(g.N / BN, g.M / BM, ks), block(256);
   if (epi == EPI_FWD)
     hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, block, 0, s, g);
   else if (epi == EPI_BWD)
     hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, block, 0, s, g);
   else
     hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, block, 0, s, g);
+  if (ks > 1) {
+    const long n4 = (long)g.M * g.N / 4;
+    hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)std::min<long>((n4 + 255) / 256, 1024)), dim3(256), 0, s,
+                       (const float4 *)g.Cpart, n4, ks, (float4 *)g.C);
+  }
   return hipGetLastError();
 }
 

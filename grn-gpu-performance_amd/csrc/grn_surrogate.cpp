@@ -79,7 +79,7 @@ int round_rows(int n) { return (n + MLP_ROW_ALIGN - 1) / MLP_ROW_ALIGN * MLP_ROW
 Surrogate::~Surrogate() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void *bufs[] = {d_p_, d_m_, d_v_, d_g_, d_wt_, d_w_, d_wout_, d_feat_, d_target_, d_x_, d_xt_, d_t_,
-                  d_mask_, d_y_, d_dy_, d_sq_, d_loss_, d_idx_};
+                  d_mask_, d_y_, d_dy_, d_sq_, d_loss_, d_idx_, d_ws_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -102,6 +102,7 @@ int Surrogate::alloc() {
     return rc;
   if ((rc = dev_alloc(&d_feat_, (size_t)p_.capacity * MLP_WIDTH)) || (rc = dev_alloc(&d_target_, p_.capacity)))
     return rc;
+  if ((rc = dev_alloc(&d_ws_, (size_t)kMaxKSplit * MLP_MAT))) return rc;
   const size_t R = rows_cap_, RW = (size_t)rows_cap_ * MLP_WIDTH;
   if ((rc = dev_alloc(&d_x_, RW)) || (rc = dev_alloc(&d_xt_, RW)) || (rc = dev_alloc(&d_t_, R)) ||
       (rc = dev_alloc(&d_mask_, R)) || (rc = dev_alloc(&d_y_, R)) || (rc = dev_alloc(&d_dy_, R)) ||
@@ -189,6 +190,11 @@ int Surrogate::backward(int rows) {
     g.N = MLP_WIDTH;
     g.K = rows;
     g.C = d_g_ + (size_t)l * MLP_MAT;
+    // 16 output tiles only: slice K so that 16 x ksplit workgroups share it
+    int ks = kMaxKSplit;
+    while (ks > 1 && (rows % (32 * ks) || rows / ks < 128)) ks /= 2;
+    g.ksplit = ks;
+    g.Cpart = d_ws_;
     SUR_TRY(grn_mlp_gemm(EPI_F32, g, stream_));
     SUR_TRY(grn_mlp_row_dot(d_dzt_[l], nullptr, MLP_WIDTH, rows, d_g_ + MLP_OFF_BIAS + (size_t)l * MLP_WIDTH,
                             stream_));

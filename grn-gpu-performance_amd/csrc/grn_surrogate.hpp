@@ -73,6 +73,9 @@ class Surrogate {
   std::vector<float> host_p_;  // initial parameters (seeded)
   // parameters and optimizer state (FP32), gradient
   float *d_p_ = nullptr, *d_m_ = nullptr, *d_v_ = nullptr, *d_g_ = nullptr;
+  // partial weight gradients of the K-sliced GEMM
+  static constexpr int kMaxKSplit = 16;
+  float *d_ws_ = nullptr;
   __bf16 *d_wt_ = nullptr, *d_w_ = nullptr, *d_wout_ = nullptr;
   // sample store
   __bf16 *d_feat_ = nullptr;
