@@ -316,8 +316,8 @@ def main():
                          "copy kernel (known size, to calibrate FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(models shard across GPUs by island, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(models would shard across GPUs by island)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "config2":

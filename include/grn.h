@@ -247,7 +247,7 @@ int grn_autotune(void *ctx, int n_models, const int *dataset_ids, const grn_mode
                  int step_cap, grn_tune_class *report, int cap, int *n_classes);
 
 /* Applies saved reports (e.g. from an earlier process): for each class the
- * candidate cand[chosen] is looked up among the configurations this build
+ * candidate cand[chosen] is looked up among the configurations this library
  * can run for the class and kept as its choice; GRN_E_INVALID if one is not
  * available.  Nothing is timed. */
 int grn_apply_tuning(void *ctx, const grn_tune_class *classes, int n);
@@ -365,8 +365,8 @@ int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, g
  * (468 features, four GELU layers of 512, one output) with BF16 weights and
  * FP32 accumulation, trained online with Adam on (model, ln error) samples
  * that the simulator produced, ranks candidate offspring so that only the
- * most promising are simulated.  The simulator stays the judge: every error
- * the search keeps is a simulated one. */
+ * most promising are simulated.  The simulator has the last word: every
+ * error the search keeps is a simulated one. */
 typedef struct {
   int n_morphogens;         /* structural genes of the encoded models */
   int batch;                /* training rows per Adam step, a multiple of 128 (4096) */
