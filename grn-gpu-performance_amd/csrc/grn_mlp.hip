@@ -40,57 +40,65 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return cdf + z * pdf;
 }
 
-// C = A . Bt^T for M % 128 == 0, N % 128 == 0, K % 32 == 0 (checked by the
-// host).  256 threads: 4 waves in a 2 x 2 arrangement, each wave a 64 x 64
-// block of sixteen 16 x 16 accumulators.
-template <int EPI>
+// C = A . Bt^T for M % TM == 0, N % 128 == 0, K % 32 == 0 (checked by the
+// host).  256 threads: 4 waves in a 2 x 2 arrangement, each wave a
+// (TM / 2) x 64 block of 16 x 16 accumulators.  TM = 128, or 64 when M is
+// small enough that 128-row tiles would leave CUs idle.
+template <int EPI, int TM>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[BM * LDS_STRIDE];
+  constexpr int WI = TM / 32;  // 16-row accumulator blocks per wave
+  __shared__ __attribute__((aligned(16))) bf16 As[TM * LDS_STRIDE];
   __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LDS_STRIDE];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * BN;
   const int K = g.K;
   // this workgroup's K slice (the whole of K unless split)
   const int ks = (EPI == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
   const int kbeg = blockIdx.z * (K / ks), kend = kbeg + K / ks;
 
-  // each thread stages 16 bf16 (two 16-byte vectors) of one A row and one Bt row
+  // each thread stages 16 bf16 (two 16-byte vectors) of one Bt row and
+  // TM / 8 bf16 of one A row (two vectors at TM = 128, one at TM = 64)
+  constexpr int AV = TM / 64;  // A vectors per thread
   const int lrow = t >> 1, lhalf = t & 1;
-  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + lrow) * K + kbeg + 16 * lhalf);
+  const int arow = t / (4 / AV), acol = 8 * AV * (t % (4 / AV));
+  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + arow) * K + kbeg + acol);
   const uint4 *pb = (const uint4 *)(g.Bt + (size_t)(n0 + lrow) * K + kbeg + 16 * lhalf);
-  uint4 ra0 = pa[0], ra1 = pa[1], rb0 = pb[0], rb1 = pb[1];
-
-  f32x4 acc[4][4];
+  uint4 ra[AV], rb0 = pb[0], rb1 = pb[1];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int v = 0; v < AV; ++v) ra[v] = pa[v];
+
+  f32x4 acc[WI][4];
+#pragma unroll
+  for (int i = 0; i < WI; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  uint4 *sa = (uint4 *)(As + lrow * LDS_STRIDE + 16 * lhalf);
+  uint4 *sa = (uint4 *)(As + arow * LDS_STRIDE + acol);
   uint4 *sb = (uint4 *)(Bs + lrow * LDS_STRIDE + 16 * lhalf);
   const int frow = lane & 15, fk = 8 * (lane >> 4);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();  // the previous tile's fragment reads are done
-    sa[0] = ra0;
-    sa[1] = ra1;
+#pragma unroll
+    for (int v = 0; v < AV; ++v) sa[v] = ra[v];
     sb[0] = rb0;
     sb[1] = rb1;
     __syncthreads();
     if (k0 + BK < kend) {  // prefetch the next K slice while this one multiplies
       pa += 4;
       pb += 4;
-      ra0 = pa[0];
-      ra1 = pa[1];
+#pragma unroll
+      for (int v = 0; v < AV; ++v) ra[v] = pa[v];
       rb0 = pb[0];
       rb1 = pb[1];
     }
-    bf16x8 af[4], bfr[4];
+    bf16x8 af[WI], bfr[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8 *)(As + (64 * wm + 16 * i + frow) * LDS_STRIDE + fk);
+    for (int i = 0; i < WI; ++i)
+      af[i] = *(const bf16x8 *)(As + ((TM / 2) * wm + 16 * i + frow) * LDS_STRIDE + fk);
 #pragma unroll
     for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8 *)(Bs + (64 * wn + 16 * j + frow) * LDS_STRIDE + fk);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WI; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
   }
@@ -98,10 +106,10 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
   const int M = g.M, N = g.N;
   float *Cout = ks > 1 ? g.Cpart + (size_t)blockIdx.z * M * N : g.C;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WI; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int mb = m0 + 64 * wm + 16 * i + 4 * (lane >> 4);  // first of this lane's 4 rows
+      const int mb = m0 + (TM / 2) * wm + 16 * i + 4 * (lane >> 4);  // first of this lane's 4 rows
       const int n = n0 + 64 * wn + 16 * j + frow;
       float v[4];
 #pragma unroll
@@ -263,13 +271,25 @@ using namespace grn_mlp;
 hipError_t grn_mlp_gemm(int epi, const MlpGemm &g, hipStream_t s) {
   const int ks = (epi == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
   if (ks > 1 && (!g.Cpart || g.K % (BK * ks))) return hipErrorInvalidValue;
-  const dim3 grid(g.N / BN, g.M / BM, ks), block(256);
-  if (epi == EPI_FWD)
-    hipLaunchKernelGGL(gemm_nt_kernel<EPI_FWD>, grid, block, 0, s, g);
-  else if (epi == EPI_BWD)
-    hipLaunchKernelGGL(gemm_nt_kernel<EPI_BWD>, grid, block, 0, s, g);
-  else
-    hipLaunchKernelGGL(gemm_nt_kernel<EPI_F32>, grid, block, 0, s, g);
+  // 64-row tiles when 128-row ones would give fewer workgroups than the
+  // device has CUs (e.g. 4096 x 512 outputs: 128 -> 256 workgroups)
+  const bool small = (long)(g.N / BN) * (g.M / BM) * ks < 256 && g.M % 64 == 0;
+  const dim3 grid(g.N / BN, g.M / (small ? 64 : BM), ks), block(256);
+  if (small) {
+    if (epi == EPI_FWD)
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_FWD, 64>), grid, block, 0, s, g);
+    else if (epi == EPI_BWD)
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_BWD, 64>), grid, block, 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 64>), grid, block, 0, s, g);
+  } else {
+    if (epi == EPI_FWD)
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_FWD, BM>), grid, block, 0, s, g);
+    else if (epi == EPI_BWD)
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_BWD, BM>), grid, block, 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, BM>), grid, block, 0, s, g);
+  }
   if (ks > 1) {
     const long n4 = (long)g.M * g.N / 4;
     hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)std::min<long>((n4 + 255) / 256, 1024)), dim3(256), 0, s,
