@@ -343,11 +343,13 @@ void Search::mutate(Rng &rng, Model *m, LabelStream *labels) {
 
 // ---- the search ---------------------------------------------------------------
 
-// Threads this process may run on (its affinity mask), at least 1.
+// Host threads for breeding: the CPUs this process may run on (its affinity
+// mask), at most 16 -- the host share that goes with one GPU.
 static int host_threads() {
   cpu_set_t set;
-  if (sched_getaffinity(0, sizeof set, &set) == 0) return std::max(1, CPU_COUNT(&set));
-  return std::max(1u, std::thread::hardware_concurrency());
+  int n = (int)std::thread::hardware_concurrency();
+  if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+  return std::max(1, std::min(16, n));
 }
 
 // f(0) ... f(n - 1) on up to host_threads() threads, dynamically scheduled.

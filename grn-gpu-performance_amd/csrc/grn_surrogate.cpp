@@ -278,7 +278,9 @@ void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vecto
 
 void Surrogate::encode_rows(const std::vector<const Model *> &models, uint16_t *rows) const {
   const size_t n = models.size();
-  const int n_threads = (int)std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), (n + 255) / 256));
+  // at most 16 threads: the host share that goes with one GPU
+  const size_t hw = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+  const int n_threads = (int)std::max<size_t>(1, std::min<size_t>(hw, (n + 255) / 256));
   auto work = [&](int t) {
     for (size_t i = t; i < n; i += n_threads) encode_model_bf16(*models[i], p_.n_morphogens, rows + i * MLP_WIDTH);
   };
