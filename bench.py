@@ -218,7 +218,7 @@ def bench_time_to_fitness(args):
         "data": "synthetic",
         "config": {"workload": "time to fitness, config 4 at reduced size: " + evolve.scaled_config4().name,
                    "seeds": seeds, "oversample": args.oversample, "screen_fraction": args.screen_fraction,
-                   "screen_policy": ["absolute", "delta"][args.screen_policy],
+                   "screen_policy": ["absolute", "delta", "better-half"][args.screen_policy],
                    "train_steps_per_generation": args.train_steps},
         "speedup_per_seed": ratios,
         "ms_per_generation": {k: float(np.mean(v)) for k, v in gen_ms.items()},
@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--models", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--seeds", default="0,1,2,3", help="evolve: comma-separated problem/search seeds")
-    ap.add_argument("--screen-policy", type=int, default=0, help="evolve: 0 absolute, 1 delta")
+    ap.add_argument("--screen-policy", type=int, default=2, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["config1", "config2", "evolve", "surrogate"], default="config1",
@@ -301,7 +301,7 @@ def main():
                          "fitness, plain vs surrogate-screened; surrogate: the MLP's training and "
                          "inference kernels")
     ap.add_argument("--generations", type=int, default=400)
-    ap.add_argument("--oversample", type=int, default=8)
+    ap.add_argument("--oversample", type=int, default=4)
     ap.add_argument("--screen-fraction", type=float, default=1.0,
                     help="evolve: fraction of each island's slots simulated per generation")
     ap.add_argument("--train-steps", type=int, default=10)

@@ -343,6 +343,14 @@ void Search::mutate(Rng &rng, Model *m, LabelStream *labels) {
 
 // ---- the search ---------------------------------------------------------------
 
+// splitmix64's finaliser: a well-mixed 64-bit hash of x.
+static uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
 // Host threads for breeding: the CPUs this process may run on (its affinity
 // mask), at most 16 -- the host share that goes with one GPU.
 static int host_threads() {
@@ -554,16 +562,26 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
     const int n = (int)kids.size();
     std::vector<int> pick(n);
     std::vector<double> gain(n);
+    std::vector<int> fresh;  // the slot's candidates that are not clones of the parent
     for (int i = 0; i < n; ++i) {
       const Individual &parent = pop[perm[i]];
       int best = -1, best_any = 0;
+      fresh.clear();
       for (int r = 0; r < oversample_; ++r) {
         const double v = sc[(size_t)r * n + i];
         if (v < sc[(size_t)best_any * n + i]) best_any = r;
         if (same_model(cand[(size_t)r * n + i].model, parent.model)) continue;
+        fresh.push_back(r);
         if (best < 0 || v < sc[(size_t)best * n + i]) best = r;
       }
       pick[i] = best < 0 ? best_any : best;
+      if (policy_ == GRN_SCREEN_HALF && fresh.size() > 1) {
+        // a uniform pick among the better half of the fresh candidates
+        std::stable_sort(fresh.begin(), fresh.end(),
+                         [&](int a, int b) { return sc[(size_t)a * n + i] < sc[(size_t)b * n + i]; });
+        const uint64_t h = mix64(p_.seed ^ mix64(((uint64_t)generation_ << 32) ^ ((uint64_t)is << 20) ^ (uint64_t)i));
+        pick[i] = fresh[h % ((fresh.size() + 1) / 2)];
+      }
       const double v = sc[(size_t)pick[i] * n + i];
       const double g = policy_ == GRN_SCREEN_DELTA ? v : v - parent.log_raw;
       gain[i] = std::isnan(g) ? std::numeric_limits<double>::infinity() : g;
@@ -981,7 +999,7 @@ int grn_search_set_screen_fraction(void *search, double fraction) {
 int grn_search_set_screen_policy(void *search, int policy) {
   SearchHandle *h = as_search(search);
   if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
-  if (policy != GRN_SCREEN_ABSOLUTE && policy != GRN_SCREEN_DELTA)
+  if (policy != GRN_SCREEN_ABSOLUTE && policy != GRN_SCREEN_DELTA && policy != GRN_SCREEN_HALF)
     return grn::set_last_error(GRN_E_INVALID, "unknown screen policy %d", policy);
   h->s->set_screen_policy(policy);
   return 0;

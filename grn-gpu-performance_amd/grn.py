@@ -59,7 +59,7 @@ MLP_FEATURES = 468
 MLP_WIDTH = 512
 
 MIGRATION_PAIRWISE, MIGRATION_RING, MIGRATION_NONE = 0, 1, 2
-SCREEN_ABSOLUTE, SCREEN_DELTA = 0, 1
+SCREEN_ABSOLUTE, SCREEN_DELTA, SCREEN_HALF = 0, 1, 2
 
 
 class GrnError(RuntimeError):

@@ -442,12 +442,18 @@ int grn_search_screen(void *search, const double *scores);
  * recorded ln raw error (grn_search_record_raw).  The other slots keep their
  * parent, as a losing child would.  Default 1. */
 int grn_search_set_screen_fraction(void *search, double fraction);
-/* What the surrogate scores in grn_search_run_generation_screened:
- * GRN_SCREEN_ABSOLUTE, a candidate's ln raw error from its own features, or
+/* What the surrogate scores in grn_search_run_generation_screened, and how a
+ * slot's child is picked:
+ * GRN_SCREEN_ABSOLUTE, a candidate's ln raw error from its own features; the
+ *   lowest score is kept;
  * GRN_SCREEN_DELTA, the change of ln raw error from its slot parent, from the
- * difference of their features and the parent's error (the surrogate then
- * learns from children with a recorded parent only).  Default absolute. */
-enum { GRN_SCREEN_ABSOLUTE = 0, GRN_SCREEN_DELTA = 1 };
+ *   difference of their features and the parent's error (the surrogate then
+ *   learns from children with a recorded parent only); the lowest is kept;
+ * GRN_SCREEN_HALF, scored as ABSOLUTE, but the child is drawn uniformly
+ *   (a seeded hash of generation, island and slot) from the better half of
+ *   the slot's non-clone candidates: the surrogate only rejects.
+ * Default absolute. */
+enum { GRN_SCREEN_ABSOLUTE = 0, GRN_SCREEN_DELTA = 1, GRN_SCREEN_HALF = 2 };
 int grn_search_set_screen_policy(void *search, int policy);
 /* Records the raw errors of the pending models (call before
  * grn_search_advance): their ln is what grn_search_set_screen_fraction

@@ -285,6 +285,29 @@ def test_screened_generations_with_a_screen_fraction():
     ctx.close()
 
 
+def test_screened_generations_with_the_better_half_policy():
+    """SCREEN_HALF: the surrogate only rejects; generations run with every
+    kept error simulated and the reported best re-simulating to itself."""
+    ctx = grn.Context(0)
+    ds = search_problem(ctx, seed=8)
+    s = grn.Search(island_datasets=[ds, ds], n_islands=2, island_size=128, seed=9, **SEARCH)
+    sur = grn.Surrogate(0, n_morphogens=3, batch=512, capacity=1 << 14, seed=3)
+    s.set_oversample(4)
+    s.set_screen_policy(grn.SCREEN_HALF)
+    g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+    prev = g["min_fit"]
+    for gen in range(4):
+        print(f"[gpu] better-half-screened generation {gen}", flush=True)
+        g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+        assert sc["n_simulated"] == 256
+        assert g["min_fit"] <= prev
+        prev = g["min_fit"]
+    best, err = s.best()
+    assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
+    sur.close()
+    ctx.close()
+
+
 def test_screened_generations_with_the_delta_policy():
     """SCREEN_DELTA: the surrogate learns the change of ln error from the slot
     parent (no samples from the unparented initial population); generations

@@ -295,3 +295,24 @@ def test_screen_fraction_without_recorded_errors_and_bad_values():
             s.set_screen_fraction(bad)
     with pytest.raises(ValueError):
         s.record_raw(np.ones(3))
+
+
+def test_screen_policy_better_half_draws_reproducibly():
+    """GRN_SCREEN_HALF: each slot's child is one of its candidates, drawn from
+    the better-scored half by a seeded hash (the same on every run), so the
+    picks differ from the lowest-score picks in some slots."""
+    def picks(policy, seed=4):
+        s = grn.Search(n_islands=2, island_size=32, seed=seed)
+        s.set_oversample(4)
+        s.advance(np.array([fake_error(m) for m in s.pending()[0]]))
+        s.set_screen_policy(policy)
+        cands = [m.to_text(17) for m in s.candidates()]
+        rng = np.random.default_rng(7)
+        s.screen(rng.permutation(len(cands)).astype(float))
+        return cands, [m.to_text(17) for m in s.pending()[0]]
+    cands, half = picks(grn.SCREEN_HALF)
+    _, half2 = picks(grn.SCREEN_HALF)
+    _, best = picks(grn.SCREEN_ABSOLUTE)
+    assert half == half2
+    assert set(half) <= set(cands) and len(half) == 64
+    assert sum(a != b for a, b in zip(half, best)) > 0
