@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--models", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--seeds", default="0,1,2,3", help="evolve: comma-separated problem/search seeds")
+    ap.add_argument("--seeds", default="0,1,2,3,4,5,6,7", help="evolve: comma-separated problem/search seeds")
     ap.add_argument("--screen-policy", type=int, default=2, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
