@@ -112,3 +112,20 @@ def test_staged_batch_equals_evaluate_batch(ctx):
     ctx.set_launch_mode(grn.LAUNCH_EAGER)
     assert np.array_equal(staged.run(), want)
     staged.close()
+
+
+def test_staged_batch_argument_errors(ctx):
+    """Error paths of the staged batch: an empty batch and a batch staged on
+    another context are rejected with the library's error, nothing runs."""
+    ids, models = mixed_batch(ctx, seed=6)
+    with pytest.raises(grn.GrnError):
+        ctx.stage_batch([], datasets=[])
+    other = grn.Context(0)
+    staged = ctx.stage_batch(models[:4], datasets=ids[:4])
+    out = np.zeros(4, grn.RESULT_DTYPE)
+    rc = other._lib.grn_run_staged(other._h, staged._h, out.ctypes.data)
+    assert rc == -1  # GRN_E_INVALID
+    want = ctx.evaluate_batch(models[:4], datasets=ids[:4])
+    assert np.array_equal(staged.run(), want)
+    staged.close()
+    other.close()
