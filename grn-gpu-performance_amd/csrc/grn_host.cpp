@@ -497,18 +497,20 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
       HIP_TRY(hipEventRecord(c->ev_fork, st));
       HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     }
-    for (size_t gi = 0; gi < P.launches.size(); ++gi) {
+    hipError_t e = hipSuccess;
+    for (size_t gi = 0; gi < P.launches.size() && e == hipSuccess; ++gi) {
       const Plan::Launch &x = P.launches[gi];
       const GrnProgram *pp = L[gi].progs;
       const GrnDataset *dp = L[gi].datasets;
       const int *op = (const int *)B.order.p;
       void *args[] = {&L[gi], &pp, &dp, &op};
-      HIP_TRY(hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, gi % 2 ? c->stream2 : st));
+      e = hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, gi % 2 ? c->stream2 : st);
     }
-    if (fork) {
+    if (fork) {  // joined also after a failed launch, so a wait on st covers both streams
       HIP_TRY(hipEventRecord(c->ev_join, c->stream2));
       HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
+    if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipLaunchKernel failed: %s", hipGetErrorString(e));
     return 0;
   };
   if (c->launch_mode == GRN_LAUNCH_GRAPH) {
