@@ -78,6 +78,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   // the planes of genes that do not diffuse stay zero (their diffusion term
   // is multiplied by dif = 0, so they must hold finite values).
   for (int i = tid; i < plane_floats; i += blockDim.x) sh.lds[i] = 0.f;
+  __syncthreads();  // zeroed before any tile stores its planes
 
   double err = 0.0, chg_acc = 0.0;
   int e = 0, steps_total = 0;
@@ -124,7 +125,8 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
               f.hi[g][k] = gidx[k] >= 0 ? vh : 0.f;
               f.lo[g][k] = gidx[k] >= 0 ? vl : 0.f;
             }
-          __syncthreads();  // every read of the previous tile's planes is done
+          // (the previous tile's last step ended with a barrier after its
+          // last plane read, so the planes can be stored at once)
           f.store_planes();
           __syncthreads();
           // (one step variant, with the per-gene diffusion branch: a second
