@@ -9,6 +9,8 @@ Tolerances (FP32 compensated state on the device vs FP64 on the host):
   * ranking of a population by fitness: identical order
 Integer outputs (steps executed, experiments run, genes in use) are exact.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -108,6 +110,28 @@ def test_grid_shapes_and_gene_counts(ctx, W, H, gr):
     ds, got, want = run_both(ctx, p, init, target, models)
     assert np.array_equal(got["n_products"], want["n_products"])
     check_fitness(got["raw_error"], want["raw_error"])
+
+
+@pytest.mark.parametrize("W,H,min_conc", [(32, 32, 0.0), (64, 64, 0.0), (48, 48, 1e-3), (64, 64, 1e-3)])
+def test_models_with_non_diffusing_genes(ctx, W, H, min_conc):
+    """Models where some genes do not diffuse run the step variant that skips
+    their diffusion terms; minConc 0 keeps them on the fast step, a positive
+    minConc (the clamp can act) on the general one.  Fitness and every field
+    agree with the oracle."""
+    p, init, target, models = population_case(71 + W + int(1e4 * min_conc), 12, W, H, 300, n_in=2,
+                                              g_range=(3, 8), l_range=(2, 20), dt=0.05)
+    p = dataclasses.replace(p, min_conc=min_conc)
+    rng = np.random.default_rng(W)
+    for m in models:  # about half the genes stop diffusing, at least one keeps it
+        off = rng.random(len(m.genes)) < 0.5
+        off[rng.integers(len(m.genes))] = False
+        m.genes["dif"] = np.where(off, 0.0, m.genes["dif"])
+    ds, got, want = run_both(ctx, p, init, target, models)
+    check_fitness(got["raw_error"], want["raw_error"])
+    for m in models[:3]:
+        f = ctx.simulate_fields(m, dataset=ds)
+        _, fo = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, want_fields=True)
+        check_fields(f, fo)
 
 
 def test_ref_fitness_with_early_stop_penalty_and_rounding(ctx):
