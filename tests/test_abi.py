@@ -142,3 +142,15 @@ def test_struct_layouts_match_header(tmp_path):
     assert grn.RESULT_DTYPE.itemsize == lay["grn_result"]
     for f in grn.RESULT_DTYPE.names:
         assert grn.RESULT_DTYPE.fields[f][1] == lay[f"grn_result.{f}"]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message, before
+    touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
