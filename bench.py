@@ -181,7 +181,7 @@ def bench_time_to_fitness(args):
             runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-4,
                                        oversample=args.oversample, train_steps=args.train_steps,
                                        screen_fraction=args.screen_fraction, screen_policy=args.screen_policy,
-                                       progress=log)
+                                       screen=args.screen, coarse_side=args.coarse_side, progress=log)
         plain, scr = runs["plain"], runs["screened"]
         thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
         table = []
@@ -191,10 +191,13 @@ def bench_time_to_fitness(args):
                           "speedup": (tp / ts) if tp and ts else None})
         tp, ts = plain.time_to(plain.best[-1]), scr.time_to(plain.best[-1])
         ratios.append((tp / ts) if tp and ts else 0.0)
-        sp += [x["spearman"] for x in scr.screen if x["n_candidates"]]
-        mae += [x["mae"] for x in scr.screen if x["n_candidates"]]
-        for k in phase:
-            phase[k] += [x[k] for x in scr.screen[2:]]
+        if args.screen == "mlp":
+            sp += [x["spearman"] for x in scr.screen if x["n_candidates"]]
+            mae += [x["mae"] for x in scr.screen if x["n_candidates"]]
+            for k in phase:
+                phase[k] += [x[k] for x in scr.screen[2:]]
+        else:
+            phase.setdefault("coarse_ms", []).extend(x["coarse_ms"] for x in scr.screen[2:])
         gen_ms["plain"].append(plain.wall[-1] / len(plain.wall) * 1e3)
         gen_ms["screened"].append(scr.wall[-1] / len(scr.wall) * 1e3)
         per_seed.append({"seed": seed, "thresholds": table,
@@ -219,10 +222,12 @@ def bench_time_to_fitness(args):
         "config": {"workload": "time to fitness, config 4 at reduced size: " + evolve.scaled_config4().name,
                    "seeds": seeds, "oversample": args.oversample, "screen_fraction": args.screen_fraction,
                    "screen_policy": ["absolute", "delta", "better-half"][args.screen_policy],
-                   "train_steps_per_generation": args.train_steps},
+                   "screen": args.screen if args.screen == "mlp" else
+                   f"coarse simulation on {args.coarse_side}x{args.coarse_side}",
+                   "train_steps_per_generation": args.train_steps if args.screen == "mlp" else None},
         "speedup_per_seed": ratios,
         "ms_per_generation": {k: float(np.mean(v)) for k, v in gen_ms.items()},
-        "screened_phase_ms": {k: float(np.mean(v)) for k, v in phase.items()},
+        "screened_phase_ms": {k: float(np.mean(v)) for k, v in phase.items() if v},
         "spearman_mean": float(np.mean(sp)) if sp else None,
         "mae_mean": float(np.mean(mae)) if mae else None,
         "per_seed": per_seed,
@@ -293,6 +298,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--seeds", default="0,1,2,3,4,5,6,7", help="evolve: comma-separated problem/search seeds")
     ap.add_argument("--screen-policy", type=int, default=2, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
+    ap.add_argument("--screen", choices=["mlp", "coarse"], default="mlp",
+                    help="evolve: score candidates with the learned surrogate or a coarse-grid simulation")
+    ap.add_argument("--coarse-side", type=int, default=8, help="evolve --screen coarse: side of the coarse grid")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["config1", "config2", "evolve", "surrogate"], default="config1",

@@ -1015,6 +1015,45 @@ int grn_search_record_raw(void *search, const double *raw_errors) {
   return 0;
 }
 
+int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_datasets, double dif_scale,
+                                     int path, grn_generation_stats *stats, double *coarse_ms) {
+  SearchHandle *h = as_search(search);
+  if (!h || !ctx || !coarse_datasets) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  if (!(dif_scale >= 0) || std::isinf(dif_scale))
+    return grn::set_last_error(GRN_E_INVALID, "dif_scale must be finite and >= 0");
+  const auto t0 = std::chrono::steady_clock::now();
+  if (h->pend.models.empty() && !h->s->candidates().empty()) {
+    const std::vector<grn::Individual *> &cl = h->s->candidates();
+    const FlatModels &c = h->candidates();
+    const int n = (int)c.models.size();
+    // the candidates with every diffusion constant scaled to the coarse grid
+    std::vector<grn_gene> genes(c.genes);
+    for (grn_gene &g : genes) g.dif *= dif_scale;
+    std::vector<grn_model> models(c.models);
+    std::vector<int> ds(n);
+    const int n_islands = h->s->params().n_islands;
+    for (int i = 0; i < n; ++i) {
+      models[i].genes = genes.data() + (c.models[i].genes - c.genes.data());
+      const int island = (int)(cl[i]->id >> 40);
+      if (island < 0 || island >= n_islands) return grn::set_last_error(GRN_E_INVALID, "candidate %d: no island", i);
+      ds[i] = coarse_datasets[island];
+    }
+    std::vector<grn_result> r(n);
+    // each candidate keeps the time step of its full-size simulation
+    const int rc = grn_evaluate_batch(ctx, n, ds.data(), models.data(), nullptr, c.dts.data(), path, r.data());
+    if (rc) return rc;
+    std::vector<double> sc(n);
+    for (int i = 0; i < n; ++i) {
+      const double e = r[i].raw_error;
+      sc[i] = std::isnan(e) ? HUGE_VAL : std::log(std::max(e, 1e-300));
+    }
+    h->s->screen(sc.data());
+    h->flatten();
+  }
+  if (coarse_ms) *coarse_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return grn_search_run_generation(search, ctx, path, stats);
+}
+
 int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int train_steps, int path,
                                        grn_generation_stats *stats, grn_screen_stats *screen) {
   SearchHandle *h = as_search(search);

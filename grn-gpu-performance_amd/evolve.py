@@ -93,25 +93,58 @@ class Run:
         return None
 
 
+def coarse_dataset(ctx, prob: Problem, side: int):
+    """The problem's experiments on a side x side grid (block means of the
+    inputs and targets) and the factor that keeps the physical diffusion:
+    (dataset id, dif_scale)."""
+    W, H = prob.params.dim_x, prob.params.dim_y
+    if W % side or H % side or W // side != H // side:
+        raise ValueError(f"a {W}x{H} grid does not coarsen to {side}x{side}")
+    f = W // side
+
+    def down(a):
+        return a.reshape(a.shape[:-2] + (a.shape[-2] // f, f, a.shape[-1] // f, f)).mean(axis=(-3, -1))
+
+    p = prob.params
+    pc = SimParams(side, side, p.n_input_morphogens, p.n_target_morphogens, p.num_sim_steps, p.dt,
+                   min_conc=p.min_conc, min_conc_change=p.min_conc_change,
+                   dist_error_threshold=p.dist_error_threshold, error_precision=p.error_precision,
+                   fitness_kind=p.fitness_kind)
+    return ctx.add_dataset(pc, down(prob.init), down(prob.target)), 1.0 / (f * f)
+
+
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
-           screen_policy: int = grn.SCREEN_ABSOLUTE, progress=None) -> Run:
+           screen_policy: int = grn.SCREEN_ABSOLUTE, screen: str = "mlp", coarse_side: int = 8,
+           progress=None) -> Run:
+    """Plain (screened False) or screened evolution.  screen "mlp": the
+    learned surrogate scores the candidates; "coarse": a simulation of every
+    candidate on a coarse_side x coarse_side grid does."""
+    if screened and screen not in ("mlp", "coarse"):
+        raise ValueError(f"unknown screen {screen!r}")
     if prob.target is None:
         make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
+    if screened and screen == "coarse":
+        ds_c, dif_scale = coarse_dataset(ctx, prob, coarse_side)
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
-    search = grn.Search(island_datasets=[ds] * prob.search["n_islands"], **prob.search)
+    n_isl = prob.search["n_islands"]
+    search = grn.Search(island_datasets=[ds] * n_isl, **prob.search)
     sur = None
     if screened:
-        sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], batch=surrogate_batch,
-                            seed=prob.search["seed"])
+        if screen == "mlp":
+            sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], batch=surrogate_batch,
+                                seed=prob.search["seed"])
         search.set_oversample(oversample)
         search.set_screen_fraction(screen_fraction)
         search.set_screen_policy(screen_policy)
     run = Run(screened)
     t0 = time.perf_counter()
     for gen in range(max_generations):
-        if screened:
+        if screened and screen == "coarse":
+            g, cms = search.run_generation_coarse(ctx, [ds_c] * n_isl, dif_scale)
+            run.screen.append({"coarse_ms": cms})
+        elif screened:
             g, sc = search.run_generation_screened(ctx, sur, train_steps=train_steps)
             run.screen.append(sc)
         else:

@@ -47,7 +47,7 @@ EXPORTED = [
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
     "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
     "grn_search_set_screen_fraction", "grn_search_set_screen_policy", "grn_search_record_raw",
-    "grn_search_run_generation_screened",
+    "grn_search_run_generation_screened", "grn_search_run_generation_coarse",
     "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
     "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
     "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
@@ -243,6 +243,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_run_generation_screened.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                                        ctypes.POINTER(_GenerationStats),
                                                        ctypes.POINTER(_ScreenStats)]
+    lib.grn_search_run_generation_coarse.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int,
+                                                     ctypes.POINTER(_GenerationStats),
+                                                     ctypes.POINTER(ctypes.c_double)]
     lib.grn_surrogate_default_params.argtypes = [ctypes.POINTER(_SurrogateParams)]
     lib.grn_surrogate_default_params.restype = None
     lib.grn_surrogate_create.argtypes = [ctypes.c_int, ctypes.POINTER(_SurrogateParams), ctypes.POINTER(vp)]
@@ -664,6 +667,19 @@ class Search:
                                                             ctypes.byref(st), ctypes.byref(sc)))
         return ({k: getattr(st, k) for k, _ in _GenerationStats._fields_},
                 {k: getattr(sc, k) for k, _ in _ScreenStats._fields_})
+
+    def run_generation_coarse(self, ctx: "Context", coarse_datasets, dif_scale: float, path: int = PATH_AUTO):
+        """(generation stats, coarse-pass ms) of one generation whose candidates
+        are screened by simulating them on coarse_datasets[island] (a coarser
+        grid of the island's experiments) with their diffusion constants times
+        dif_scale; see grn_search_run_generation_coarse."""
+        ds = np.ascontiguousarray(coarse_datasets, dtype=np.int32)
+        if ds.shape != (self.params["n_islands"],):
+            raise ValueError(f"expected {self.params['n_islands']} coarse dataset ids")
+        st, ms = _GenerationStats(), ctypes.c_double()
+        _check(self._lib.grn_search_run_generation_coarse(self._h, ctx._h, ds.ctypes.data, float(dif_scale), path,
+                                                          ctypes.byref(st), ctypes.byref(ms)))
+        return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}, ms.value
 
     def best(self):
         """(Model, error) of the lowest-error individual (ties: lower complexity)."""

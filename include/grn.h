@@ -466,6 +466,19 @@ int grn_search_record_raw(void *search, const double *raw_errors);
  * and return.  The initial population is simulated unscreened. */
 int grn_search_run_generation_screened(void *search, void *ctx, void *surrogate, int train_steps, int path,
                                        grn_generation_stats *stats, grn_screen_stats *screen);
+/* One generation screened by a coarse simulation instead of the learned
+ * surrogate (a multi-fidelity screen; the reference has no counterpart, its
+ * search simulates every child): every candidate is simulated on
+ * coarse_datasets[island] -- the island's experiments on a coarser grid --
+ * with each diffusion constant multiplied by dif_scale (for a grid coarser
+ * by a factor f in both directions, 1/f^2 keeps the physical diffusion) and
+ * its full-size time step; its ln raw error is the score grn_search_screen
+ * ranks (under the screen policy); the kept children are then simulated on
+ * their own datasets and the search advances.  The initial population is
+ * simulated unscreened.  coarse_ms (optional) receives the wall time of the
+ * coarse pass and the screen. */
+int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_datasets, double dif_scale,
+                                     int path, grn_generation_stats *stats, double *coarse_ms);
 
 #ifdef __cplusplus
 }

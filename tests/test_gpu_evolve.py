@@ -24,3 +24,68 @@ def test_plain_and_screened_evolution_are_verified_and_deterministic():
     assert s.best[0] == a.best[0]  # the same initial population
     assert all(sc["n_candidates"] in (0, 4 * 512) for sc in s.screen)
     ctx.close()
+
+
+def test_coarse_screen_equals_its_composition_and_ranks_candidates():
+    """grn_search_run_generation_coarse keeps the children the public calls
+    pick (candidates, a coarse simulation with scaled diffusion, screen),
+    and the coarse errors rank the candidates like the full-size ones."""
+    from scipy.stats import spearmanr
+    print("\n[gpu] coarse screen", flush=True)
+    ctx = grn.Context(0)
+    prob = evolve.scaled_config4(seed=1, n_islands=2, island_size=64, steps=300)
+    evolve.make_target(ctx, prob)
+    ds = ctx.add_dataset(prob.params, prob.init, prob.target)
+    ds_c, scale = evolve.coarse_dataset(ctx, prob, 8)
+    assert scale == 1.0 / 16
+    a = grn.Search(island_datasets=[ds, ds], **prob.search)
+    b = grn.Search(island_datasets=[ds, ds], **prob.search)
+    for s in (a, b):
+        s.set_oversample(4)
+        s.set_screen_policy(grn.SCREEN_ABSOLUTE)
+    rhos = []
+    for gen in range(3):
+        a.run_generation_coarse(ctx, [ds_c, ds_c], scale)
+        c = b.candidates()
+        if c:
+            dts = []
+            for m in c:
+                dmax = float(np.max(m.genes["dif"]))
+                dts.append(min(prob.search["dt_scale"] / dmax, prob.search["dt_cap"]) if dmax > 0
+                           else prob.search["dt_cap"])
+            cm = []
+            for m in c:
+                g = m.genes.copy()
+                g["dif"] = g["dif"] * scale
+                cm.append(grn.Model(g, m.links))
+            coarse = ctx.evaluate_batch(cm, datasets=[ds_c] * len(c), dt=dts)["raw_error"]
+            fine = ctx.evaluate_batch(c, datasets=[ds] * len(c), dt=dts)["raw_error"]
+            rhos.append(spearmanr(coarse, fine).correlation)
+            b.screen(np.where(np.isnan(coarse), np.inf, np.log(np.maximum(coarse, 1e-300))))
+        b.run_generation(ctx)
+        pa, pb = a.pending(), b.pending()
+        assert [m.to_text(17) for m in pa[0]] == [m.to_text(17) for m in pb[0]], gen
+    print(f"[gpu] coarse/fine Spearman per generation: {rhos}", flush=True)
+    assert len(rhos) == 2 and min(rhos) > 0.6
+    a.close()
+    b.close()
+    ctx.close()
+
+
+def test_coarse_screened_evolution_is_verified_and_deterministic():
+    print("\n[gpu] coarse-screened evolution", flush=True)
+    ctx = grn.Context(0)
+    prob = evolve.scaled_config4(seed=2, n_islands=4, island_size=128, steps=400)
+    evolve.make_target(ctx, prob)
+    kw = dict(max_generations=12, threshold=0.0, oversample=4, screen="coarse", coarse_side=8,
+              screen_policy=grn.SCREEN_ABSOLUTE)
+    a = evolve.evolve(ctx, prob, True, **kw)
+    b = evolve.evolve(ctx, prob, True, **kw)
+    assert a.best == b.best and a.verified and b.verified
+    assert all(x >= y for x, y in zip(a.best, a.best[1:]))
+    p = evolve.evolve(ctx, prob, False, max_generations=1, threshold=0.0)
+    assert a.best[0] == p.best[0]  # the same initial population, simulated unscreened
+    assert all(x["coarse_ms"] >= 0 for x in a.screen)
+    with pytest.raises(ValueError):
+        evolve.evolve(ctx, prob, True, max_generations=1, threshold=0.0, screen="coarse", coarse_side=12)
+    ctx.close()

@@ -316,3 +316,23 @@ def test_screen_policy_better_half_draws_reproducibly():
     assert half == half2
     assert set(half) <= set(cands) and len(half) == 64
     assert sum(a != b for a, b in zip(half, best)) > 0
+
+
+def test_coarse_generation_rejects_bad_arguments():
+    # argument checks run before any device work (no context is needed);
+    # -1 is GRN_E_INVALID
+    import ctypes
+    s = grn.Search(n_islands=2, island_size=4)
+    lib = grn.load_library()
+    ds = np.zeros(2, np.int32)
+    assert lib.grn_search_run_generation_coarse(s._h, None, ds.ctypes.data, 0.25, grn.PATH_AUTO, None, None) \
+        == -1
+    assert lib.grn_search_run_generation_coarse(None, None, ds.ctypes.data, 0.25, grn.PATH_AUTO, None, None) \
+        == -1
+    fake_ctx = ctypes.c_void_p(8)  # never dereferenced: the scale check comes first
+    for bad in (-1.0, float("inf"), float("nan")):
+        assert lib.grn_search_run_generation_coarse(s._h, fake_ctx, ds.ctypes.data, bad, grn.PATH_AUTO,
+                                                    None, None) == -1
+    with pytest.raises(ValueError):
+        s.run_generation_coarse(None, [0, 0, 0], 0.25)  # one coarse dataset per island
+    s.close()
