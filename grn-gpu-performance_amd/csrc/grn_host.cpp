@@ -196,10 +196,14 @@ bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int 
 // order the default heuristic prefers them: on-chip kernels (exact width,
 // then any width) unless the tiled path is forced, then the halo-tiled
 // kernels with halo widths 4, 2 and 8 (512-thread instantiations first:
-// they keep their state in registers without spilling).
+// they keep their state in registers without spilling).  Of two on-chip
+// kernels of one pass that launch the same threads (a grid of at most 64
+// cells is one wavefront either way), the one with fewer cells per thread
+// comes first: the same wavefronts, less serial work per Euler step.
 std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path) {
   std::vector<KernelChoice> out;
   for (int pass = 0; pass < 2 && path != GRN_PATH_TILED; ++pass) {  // exact width first, then any width
+    const size_t pass_begin = out.size();
     for (const GrnKernelEntry *k : c.kernels) {
       if (pass == 0 ? k->wc != W : k->wc != 0) continue;
       if (G < k->glo || G > k->ghi || k->guard != guard) continue;
@@ -215,7 +219,13 @@ std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, i
       kc.k = k;
       kc.block = block;
       kc.lds = lds;
-      out.push_back(kc);
+      size_t at = out.size();
+      for (size_t j = pass_begin; j < out.size(); ++j)
+        if (out[j].block == block && out[j].k->cpt > k->cpt) {
+          at = j;
+          break;
+        }
+      out.insert(out.begin() + (long)at, kc);
     }
   }
   if (path != GRN_PATH_LDS) {
