@@ -126,6 +126,11 @@ struct Ctx {
   std::map<TuneKey, KernelChoice> tuned;               // autotuner's choices
   mutable std::map<std::pair<TuneKey, int>, KernelChoice> defaults;  // default choice per class and path
   grn_stats stats{};
+  // Pinned host staging of the program uploads (a batch's largest upload:
+  // 2 KB per model).  Every batch that fills it waits for its stream before
+  // returning, so the next batch can refill it.
+  GrnProgram *h_progs = nullptr;
+  size_t h_progs_cap = 0;
 
   void drop_graphs() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -144,6 +149,7 @@ struct Ctx {
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
+    if (h_progs) (void)hipHostFree(h_progs);
   }
 };
 
@@ -441,8 +447,10 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
 }
 
 // Sizes the buffers and enqueues the uploads of programs, order, maxError
-// bounds and the batch table.  The host arrays are the plan's own.
-int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<GrnProgram> *progs, std::vector<double> *maxerr) {
+// bounds and the batch table.  The programs go through the context's pinned
+// staging; the other host arrays are the plan's own.  The caller waits for
+// the stream before the staging or those arrays are reused.
+int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr) {
   const int n = P.n;
   int rc;
   if ((rc = B.progs.reserve(sizeof(GrnProgram) * std::max(n, 1)))) return rc;
@@ -451,15 +459,23 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<GrnProgram> *pr
   if ((rc = B.out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
   if ((rc = B.batch.reserve(sizeof(GrnBatch)))) return rc;
   if (P.scratch_bytes && (rc = B.scratch.reserve(P.scratch_bytes))) return rc;
-  progs->resize(n);
+  hipStream_t st = c->stream;
+  if ((size_t)n > c->h_progs_cap) {
+    HIP_TRY(hipStreamSynchronize(st));  // no upload still reads the old staging
+    if (c->h_progs) (void)hipHostFree(c->h_progs);
+    c->h_progs = nullptr;
+    c->h_progs_cap = 0;
+    const size_t cap = std::max<size_t>((size_t)n + n / 2, 1024);
+    HIP_TRY(hipHostMalloc((void **)&c->h_progs, sizeof(GrnProgram) * cap, hipHostMallocDefault));
+    c->h_progs_cap = cap;
+  }
   maxerr->resize(n);
   for (int i = 0; i < n; ++i) {
-    (*progs)[i] = P.items[i].prog;
+    c->h_progs[i] = P.items[i].prog;
     (*maxerr)[i] = P.items[i].max_err;
   }
-  hipStream_t st = c->stream;
   if (n > 0) {
-    HIP_TRY(hipMemcpyAsync(B.progs.p, progs->data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(B.progs.p, c->h_progs, sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(B.order.p, P.idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(B.maxerr.p, maxerr->data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   }
@@ -627,10 +643,9 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
     if (P.groups.size() > 1) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
   }
-  std::vector<GrnProgram> progs;
   std::vector<double> maxerr;
   bool replayed = false;
-  if ((rc = upload_plan(c, P, c->bufs, &progs, &maxerr)) ||
+  if ((rc = upload_plan(c, P, c->bufs, &maxerr)) ||
       (rc = launch_plan(c, P, c->bufs, fields_host ? &c->d_fields : nullptr, field_stride, field_exp, &replayed))) {
     (void)hipStreamSynchronize(c->stream);  // the uploads read host buffers of this frame
     return rc;
@@ -659,8 +674,7 @@ struct Staged {
   Ctx *c = nullptr;
   Plan P;
   BatchBufs B;
-  std::vector<GrnProgram> progs;  // sources of the uploads
-  std::vector<double> maxerr;
+  std::vector<double> maxerr;  // a source of the uploads
 };
 
 }  // namespace
@@ -815,8 +829,8 @@ int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_m
   std::unique_ptr<Staged> S(new Staged);
   S->c = c;
   int rc = make_plan(c, n_models, dataset_ids, models, max_errors, dt_override, path, BatchOptions(), &S->P);
-  if (!rc) rc = upload_plan(c, S->P, S->B, &S->progs, &S->maxerr);
-  const hipError_t e = hipStreamSynchronize(c->stream);  // the uploads read S's host arrays
+  if (!rc) rc = upload_plan(c, S->P, S->B, &S->maxerr);
+  const hipError_t e = hipStreamSynchronize(c->stream);  // the uploads read S's and the context's host arrays
   if (rc) return rc;
   if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
   *staged = S.release();
