@@ -842,12 +842,17 @@ struct SearchHandle {
 
   bool cand_built = false;
   const void *rows_lender = nullptr;  // the surrogate whose staging rows the last breeding wrote
+  // the coarse screen's arrays (kept, with their capacity, across generations)
+  std::vector<grn_gene> coarse_genes;
+  std::vector<grn_model> coarse_models;
+  std::vector<int> coarse_ds;
+  std::vector<grn_result> coarse_res;
+  std::vector<double> coarse_scores;
 
   // pending() is flattened at once; candidates() only when asked for
   void flatten() {
     pend.build(s->pending());
-    cand = FlatModels();
-    cand_built = false;
+    cand_built = false;  // (its arrays keep their capacity for the next candidates)
   }
   const FlatModels &candidates() {
     if (!cand_built) cand.build(s->candidates());
@@ -1027,10 +1032,13 @@ int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_
     const FlatModels &c = h->candidates();
     const int n = (int)c.models.size();
     // the candidates with every diffusion constant scaled to the coarse grid
-    std::vector<grn_gene> genes(c.genes);
+    std::vector<grn_gene> &genes = h->coarse_genes;
+    std::vector<grn_model> &models = h->coarse_models;
+    std::vector<int> &ds = h->coarse_ds;
+    genes.assign(c.genes.begin(), c.genes.end());
     for (grn_gene &g : genes) g.dif *= dif_scale;
-    std::vector<grn_model> models(c.models);
-    std::vector<int> ds(n);
+    models.assign(c.models.begin(), c.models.end());
+    ds.resize(n);
     const int n_islands = h->s->params().n_islands;
     for (int i = 0; i < n; ++i) {
       models[i].genes = genes.data() + (c.models[i].genes - c.genes.data());
@@ -1038,11 +1046,13 @@ int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_
       if (island < 0 || island >= n_islands) return grn::set_last_error(GRN_E_INVALID, "candidate %d: no island", i);
       ds[i] = coarse_datasets[island];
     }
-    std::vector<grn_result> r(n);
+    std::vector<grn_result> &r = h->coarse_res;
+    r.resize(n);
     // each candidate keeps the time step of its full-size simulation
     const int rc = grn_evaluate_batch(ctx, n, ds.data(), models.data(), nullptr, c.dts.data(), path, r.data());
     if (rc) return rc;
-    std::vector<double> sc(n);
+    std::vector<double> &sc = h->coarse_scores;
+    sc.resize(n);
     for (int i = 0; i < n; ++i) {
       const double e = r[i].raw_error;
       sc[i] = std::isnan(e) ? HUGE_VAL : std::log(std::max(e, 1e-300));
