@@ -380,6 +380,26 @@ int gro_evaluate(const gro_params *p, int n_exp, const double *init,
                      links, max_error, out, final_fields);
 }
 
+/* ---- experiment images (simstate.cpp:71-90) ----------------------------- */
+
+int gro_decode_rgb(int width, int height, const unsigned int *rgb, int n_planes,
+                   double *out) {
+  if (width < 1 || height < 1 || n_planes < 0 || n_planes > 3) return -1;
+  /* SimState::loadMorphologyImage: scan line j of the image is row
+   * height - 1 - j of the planes (Qt's y axis grows down), and plane k takes
+   * byte 2 - k of the QRgb (k = 0 red, 1 green, 2 blue) divided by 255. */
+  for (int j = 0; j < height; ++j) {
+    const unsigned int *line = rgb + (size_t)j * width;
+    for (int i = 0; i < width; ++i) {
+      unsigned int color = line[i];
+      for (int k = 0; k < n_planes; ++k)
+        out[((size_t)k * height + (size_t)(height - j - 1)) * width + i] =
+            ((double)((color >> (16 - (8 * k))) & 0xff)) / 255.0;
+    }
+  }
+  return 0;
+}
+
 /* ---- model-parallel batch ---------------------------------------------- */
 
 typedef struct {

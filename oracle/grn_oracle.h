@@ -16,6 +16,7 @@
  *   log fitness ........... Src/Simulator/simstate.cpp:135-158
  *   evaluate contract ..... Src/Search/evaluatormorphogens.cpp:48-76
  *   ceilS, min, max ....... Src/Common/mathalgo.h:55-62,208-216
+ *   experiment images ..... Src/Simulator/simstate.cpp:71-90
  *
  * Field layout used by every function here: plane k, row y, column x at
  * k*H*W + y*W + x (x fastest), which is the element order of the reference's
@@ -99,6 +100,14 @@ int gro_evaluate_batch_dt(const gro_params *p, int n_exp, const double *init,
                           const int *link_off, const gro_link *links,
                           const double *max_error, const double *dt,
                           int n_threads, gro_result *out);
+
+/* SimState::loadMorphologyImage (simstate.cpp:71-90): decodes n_planes
+ * (<= 3) channels of a width x height image of packed 0xAARRGGBB pixels,
+ * scan lines top to bottom, into planes out[n_planes][height][width] (x
+ * fastest): plane k is byte 2 - k / 255, image line j is plane row
+ * height - 1 - j.  Returns 0, or -1 for bad sizes. */
+int gro_decode_rgb(int width, int height, const unsigned int *rgb, int n_planes,
+                   double *out);
 
 #ifdef __cplusplus
 }

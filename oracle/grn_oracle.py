@@ -61,6 +61,7 @@ def load():
         lib.gro_evaluate_batch_dt.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, vp, vp,
                                               ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.c_int,
                                               vp]
+        lib.gro_decode_rgb.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp]
         _lib = lib
     return _lib
 
@@ -149,4 +150,20 @@ def evaluate_batch(p: _Params, init, target, models, max_errors=None, dt=None, n
                                       out.ctypes.data)
     if rc:
         raise ValueError(f"oracle rejected a model ({rc})")
+    return out
+
+
+def decode_rgb(images, n_planes):
+    """SimState::loadMorphologyImage (simstate.cpp:71-90) of each image in
+    images[n][H][W] (uint32 0xAARRGGBB, scan lines top to bottom) ->
+    planes [n][n_planes][H][W] (float64)."""
+    img = np.ascontiguousarray(images, np.uint32)
+    n, H, W = img.shape
+    out = np.zeros((n, n_planes, H, W), np.float64)
+    for e in range(n):
+        o = np.zeros((n_planes, H, W), np.float64)
+        rc = load().gro_decode_rgb(W, H, img[e].ctypes.data, n_planes, o.ctypes.data)
+        if rc:
+            raise ValueError("bad image size or plane count")
+        out[e] = o
     return out

@@ -10,8 +10,13 @@ mathematics fixes, independently of the oracle's own code:
   (pins the neighbour sums and the no-flux boundary);
 * the stop rule / penalty / sentinel / ceilS contract
   (simulator.cpp:54-63, evaluatormorphogens.cpp:48-76, mathalgo.h:55-62);
+* the minConc clamp, the rate-zero skip and the diffusion guard of the
+  update (modelsimulator.cpp:255,263-275), each with a case a flipped
+  comparison would fail;
 * invariances: relabelling free genes, genes outside the in-use closure,
-  mirror symmetry.
+  mirror symmetry;
+* the experiment-image decode (simstate.cpp:71-90) on pixels with known
+  channel bytes.
 """
 import math
 
@@ -160,6 +165,81 @@ def test_diffusion_neumann_eigenmode(k, l):
     np.testing.assert_allclose(f[0], A + B * lam ** S * mode, rtol=1e-11, atol=1e-13)
 
 
+# ---- the update's branches: minConc clamp, rate-zero skip, diffusion guard -------
+
+def test_min_conc_clamp_is_strict_and_then_holds_zero():
+    # c0 = 1, dt*deg = 1/2: c_s = 2^-s exactly in binary.  With minConc =
+    # 1/8 the value 1/8 itself stays (c < minConc is strict) and the next
+    # step (1/16 < 1/8) stores 0, which then stays 0 (R = -deg*0 = 0).
+    W, H, dt, deg = 3, 2, 0.5, 1.0
+    for S, want in ((2, 0.25), (3, 0.125), (4, 0.0), (9, 0.0)):
+        p = ora.params(W, H, 1, 1, S, dt, min_conc=0.125)
+        r, f = run(p, uniform(W, H, 1.0), uniform(W, H, 0.0), genes((0, 0, deg, 0), (1, 0, 0, 0)), links())
+        assert r["steps"] == S
+        assert np.all(f[0] == want), (S, f[0])
+
+
+def test_stop_rule_after_clamp_to_zero():
+    # minConcChange = 0: steps 1..4 change by deg*c = 1, 1/2, 1/4, 1/8 > 0;
+    # step 4 clamps to 0, so step 5 has R = 0 everywhere, its change is 0
+    # and the loop ends (a >= in the stop test would run all S steps).
+    p = ora.params(3, 2, 1, 1, 100, 0.5, min_conc=0.125, min_conc_change=0.0, fitness_kind=ora.FITNESS_REF)
+    r, f = run(p, uniform(3, 2, 1.0), uniform(3, 2, 0.0), genes((0, 0, 1.0, 0), (1, 0, 0, 0)), links())
+    assert r["steps"] == 5 and r["change"] == 0.0
+    assert np.all(f[0] == 0.0)
+
+
+def test_min_conc_clamp_closed_form_crossing():
+    # c_s = c0 (1 - dt deg)^s while >= minConc; the first step below it
+    # stores 0.  s* = first s with c0 (1 - dt deg)^s < minConc.
+    W, H, dt, deg, c0, mc = 2, 2, 0.01, 0.2, 0.8, 0.5
+    s_star = math.ceil(math.log(mc / c0) / math.log(1 - dt * deg))
+    assert c0 * (1 - dt * deg) ** (s_star - 1) >= mc > c0 * (1 - dt * deg) ** s_star
+    for S, want in ((s_star - 1, c0 * (1 - dt * deg) ** (s_star - 1)), (s_star, 0.0), (s_star + 7, 0.0)):
+        p = ora.params(W, H, 1, 1, S, dt, min_conc=mc)
+        r, f = run(p, uniform(W, H, c0), uniform(W, H, 0.0), genes((0, 0, deg, 0), (1, 0, 0, 0)), links())
+        np.testing.assert_allclose(f[0], want, rtol=1e-12, atol=0)
+
+
+def test_rate_zero_cell_is_not_clamped():
+    # A gene with lim = deg = dif = 0 has R = 0 everywhere: the update is
+    # skipped (modelsimulator.cpp:265), so a value below minConc survives;
+    # without the skip the clamp would store 0.
+    p = ora.params(3, 3, 1, 1, 10, 0.1, min_conc=0.1)
+    r, f = run(p, uniform(3, 3, 0.05), uniform(3, 3, 0.0), genes((0, 0, 0, 0), (1, 0, 0, 0)), links())
+    assert np.all(f[0] == 0.05)
+
+
+def test_diffusion_guard_brute_force_3x3():
+    # One Euler step of a diffusing input gene (lim = deg = 0, dif = 0.5,
+    # dt = 0.1, minConc = -10 so nothing is clamped) on a 3x3 field with
+    # zero and negative values.  Per cell, from the step-start field: S = sum
+    # of the existing 4-neighbours, n = their count; diffusion applies only
+    # when S > 0 or (n > 0 and c > 0) (modelsimulator.cpp:255), and then
+    # c' = c + dt*dif*(S - n*c) = c + 0.05*(S - n*c); otherwise R = 0 and the
+    # cell keeps its value.  Expected values, worked by hand:
+    #   (y,x)  c      S      n  guard  c'
+    #   (0,0)  0     -0.5    2  off    0       (S < 0, c = 0: c >= 0 would give -0.025)
+    #   (0,1)  0.5   -0.5    3  on     0.4     (0.5 + 0.05*(-0.5 - 1.5))
+    #   (0,2) -0.5    1.0    2  on    -0.4     (-0.5 + 0.05*(1.0 + 1.0))
+    #   (1,0) -1.0    0.0    3  off   -1.0     (S = 0, c < 0: S >= 0 would give -0.85)
+    #   (1,1)  0     -0.5    4  off    0
+    #   (1,2)  0.5   -0.75   3  on     0.3875  (0.5 + 0.05*(-0.75 - 1.5))
+    #   (2,0)  0     -1.5    2  off    0
+    #   (2,1) -0.5   -0.25   3  off   -0.5     (no guard at all would give -0.4375)
+    #   (2,2) -0.25   0.0    2  off   -0.25
+    c = np.array([[0.0, 0.5, -0.5], [-1.0, 0.0, 0.5], [0.0, -0.5, -0.25]])
+    want = np.array([[0.0, 0.4, -0.4], [-1.0, 0.0, 0.3875], [0.0, -0.5, -0.25]])
+    p = ora.params(3, 3, 1, 1, 1, 0.1, min_conc=-10.0)
+    r, f = run(p, c[None, None], np.zeros((1, 1, 3, 3)), genes((0, 0, 0, 0.5), (1, 0, 0, 0)), links())
+    np.testing.assert_allclose(f[0], want, rtol=1e-15, atol=1e-16)
+    # the guard is read from the step-start field: a second step continues
+    # from these values (cell (1,0) now sees S = 0 + 0 + 0 = 0 and c = -1: off)
+    p2 = ora.params(3, 3, 1, 1, 2, 0.1, min_conc=-10.0)
+    r2, f2 = run(p2, c[None, None], np.zeros((1, 1, 3, 3)), genes((0, 0, 0, 0.5), (1, 0, 0, 0)), links())
+    assert f2[0, 1, 0] == -1.0
+
+
 # ---- contract: stop rule, penalty, sentinel, ceilS -----------------------------
 
 def test_stop_rule_counts_steps():
@@ -306,3 +386,27 @@ def test_batch_matches_single_model_calls():
     batch = ora.evaluate_batch(p, w.init, tgt, w.models, n_threads=3)
     single = [ora.evaluate(p, w.init, tgt, m.genes, m.links)[0]["raw_error"] for m in w.models]
     assert np.array_equal(batch["raw_error"], single)
+
+
+# ---- experiment images -------------------------------------------------------------
+
+def test_decode_rgb_channels_and_y_flip():
+    # SimState::loadMorphologyImage (simstate.cpp:71-90): plane k is byte
+    # 2 - k of the pixel (red, green, blue) / 255; image scan line j (top
+    # first) lands in plane row H - 1 - j.
+    W, H = 3, 2
+    img = np.array([[[0xFF102030, 0x00FF0000, 0x0000FF00],
+                     [0x000000FF, 0x00808080, 0xFFFFFFFF]]], np.uint32)
+    out = ora.decode_rgb(img, 3)
+    assert out.shape == (1, 3, H, W)
+    # top scan line -> row 1, bottom scan line -> row 0
+    np.testing.assert_array_equal(out[0, :, 1, 0], [0x10 / 255, 0x20 / 255, 0x30 / 255])
+    np.testing.assert_array_equal(out[0, :, 1, 1], [1.0, 0.0, 0.0])
+    np.testing.assert_array_equal(out[0, :, 1, 2], [0.0, 1.0, 0.0])
+    np.testing.assert_array_equal(out[0, :, 0, 0], [0.0, 0.0, 1.0])
+    np.testing.assert_array_equal(out[0, :, 0, 1], [128 / 255] * 3)
+    np.testing.assert_array_equal(out[0, :, 0, 2], [1.0, 1.0, 1.0])
+    # fewer planes read only the leading channels; the alpha byte is ignored
+    np.testing.assert_array_equal(ora.decode_rgb(img, 1)[0, 0], out[0, 0])
+    with pytest.raises(ValueError):
+        ora.decode_rgb(img, 4)
