@@ -43,8 +43,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, budget_s=22.0):
-    """FP64 oracle on the host cores over a bounded sample of the workload."""
+def cpu_baseline(w, gpu_res=None, budget_s=22.0):
+    """FP64 oracle on the host cores over a bounded sample of the workload.
+    gpu_res (optional): the device's results for every model of the workload,
+    compared with the oracle's on the timed sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import grn_oracle as ora
 
@@ -63,6 +65,14 @@ def cpu_baseline(w, budget_s=22.0):
     r = ora.evaluate_batch(p, w.init, w.target, sample, n_threads=threads)
     dt = time.perf_counter() - t0
     cs = float(np.sum(r["steps"]) * cells)
+    parity = None
+    if gpu_res is not None:
+        g = gpu_res[len(probe):len(probe) + n]
+        rel = np.abs(g["raw_error"] - r["raw_error"]) / np.abs(r["raw_error"])
+        order_same = bool(np.array_equal(np.argsort(g["raw_error"], kind="stable"),
+                                         np.argsort(r["raw_error"], kind="stable")))
+        parity = {"models": int(n), "max_rel_fitness": float(rel.max()), "steps_equal":
+                  bool(np.array_equal(g["steps"], r["steps"])), "ranking_identical": order_same}
     # one thread on three more models
     one = w.models[len(probe) + n:len(probe) + n + 3]
     t1 = time.perf_counter()
@@ -78,7 +88,8 @@ def cpu_baseline(w, budget_s=22.0):
             "sample": f"{len(sample)} of the {len(w.models)} models at full 2000 steps, FP64, "
                       f"{threads} threads, {dt:.1f} s",
             "single_thread_value": float(np.sum(r1["steps"]) * cells) / dt1,
-            "single_thread_sample": f"{len(one)} models, {dt1:.1f} s", "cpu_model": cpu_model}
+            "single_thread_sample": f"{len(one)} models, {dt1:.1f} s", "cpu_model": cpu_model,
+            "parity_vs_gpu": parity}
 
 
 def bench_islands(args):
@@ -403,7 +414,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
-        cpu = cpu_baseline(w)
+        cpu = cpu_baseline(w, gpu_res=res)
 
     out = {
         "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
@@ -433,6 +444,7 @@ def main():
         "plain": {"value": plain_rate, "unit": "cell-steps/s", "models": len(sample),
                   "pattern": "one model per launch, synchronise per model"},
         "batched_over_plain": value / plain_rate if plain_rate else None,
+        "parity_max_rel": cpu["parity_vs_gpu"]["max_rel_fitness"] if cpu else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

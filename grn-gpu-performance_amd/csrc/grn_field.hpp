@@ -333,11 +333,11 @@ struct Model {
   // last step's max change exceeds minConcChange.  A negative threshold can
   // never stop the loop, so then only the final step needs the reduction.
   template <bool FAST, bool MASKED>
-  __device__ __forceinline__ float run_steps(int S, float mcc, int *steps_done) {
+  __device__ __forceinline__ float run_steps(int S, double mcc, int *steps_done) {
     float chg = 1.0f;
     int s = 0;
-    if (mcc >= 0.f) {
-      while (s < S && chg > mcc) {
+    if (mcc >= 0.0) {
+      while (s < S && (double)chg > mcc) {
         chg = step<true, FAST, MASKED>();
         ++s;
       }
@@ -355,13 +355,18 @@ struct Model {
 
 // Sum of the per-cell fitness terms over this thread's share of the staged
 // final planes: REF terms log(1 + (|c - T| - thr)) where positive
-// (simstate.cpp:135-158), MSE terms (c - T)^2.  Kept out of line so its
-// double-precision log does not share a register allocation with the
-// simulation loop.
+// (simstate.cpp:135-158), MSE terms (c - T)^2.  The terms are dealt to the
+// first nb threads (the model's own thread count, rounded to whole waves),
+// not to the whole workgroup, so the order of the FP64 sum -- and with it
+// every bit of the fitness -- does not depend on the workgroup size a batch
+// gives the model; threads beyond nb contribute exact zeros.  Kept out of
+// line so its double-precision log does not share a register allocation
+// with the simulation loop.
 __device__ __noinline__ double score_terms(const double *stage, const double *target, int count,
-                                           double thr, int mse) {
+                                           double thr, int mse, int nb) {
   double acc = 0.0;
-  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+  if ((int)threadIdx.x >= nb) return acc;
+  for (int i = threadIdx.x; i < count; i += nb) {
     const double d = stage[i] - target[i];
     if (mse) {
       acc += d * d;
@@ -406,9 +411,10 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
   }
   const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt;
   const int S = (a.step_cap > 0 && a.step_cap < D->steps) ? a.step_cap : D->steps;
-  const float mcc = D->min_conc_change;
+  const double mcc = D->min_conc_change_d;
   const double max_err = a.max_err[pi];
   const int plane_floats = G * f.plane;
+  const int nb = (n_active + 63) & ~63;  // the model's own thread count in whole waves
 
   double err = 0.0, chg_acc = 0.0;
   int e = 0, steps_total = 0;
@@ -470,20 +476,20 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
     // REF: sum of log(1 + (|c - T| - thr)) over positive terms, / (W*H)
     // (simstate.cpp:135-158).  MSE: mean of (c - T)^2 over cells and targets.
     const bool mse = D->kind == 1;
-    double acc = score_terms(stage, D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr, mse ? 1 : 0);
+    double acc = score_terms(stage, D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr, mse ? 1 : 0, nb);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) sh.wsum[wave] = acc;
     __syncthreads();
     double tot = 0.0;
-    for (int w = 0; w < nw; ++w) tot += sh.wsum[w];
+    for (int w = 0; w < (nb >> 6); ++w) tot += sh.wsum[w];
 
     double e_err;
     if (mse) {
       e_err = tot / ((double)N * n_tgt);
     } else {
       e_err = tot / N;
-      if (chg_acc > (double)mcc) e_err += chg_acc - (double)mcc;  // evaluatormorphogens.cpp:62-63
+      if (chg_acc > mcc) e_err += chg_acc - mcc;  // evaluatormorphogens.cpp:62-63
     }
     err += e_err / n_exp;
     ++e;

@@ -18,6 +18,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -104,7 +106,7 @@ struct TuneKey {
 // The device buffers of a batch: the context's own (grow-only, reused by
 // every grn_evaluate_batch) or a staged batch's.
 struct BatchBufs {
-  DevBuf progs, order, maxerr, out, batch, scratch;
+  DevBuf progs, progs64, order, maxerr, out, batch, scratch;
 };
 
 struct Ctx {
@@ -157,7 +159,8 @@ void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
   const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w32wide, grn_field_table_w32wide9,
                                               grn_field_table_w32wide13, grn_field_table_w48,
                                               grn_field_table_w64, grn_field_table_wany, grn_field_table_wanyguard,
-                                              grn_field_table_tiled512, grn_field_table_tiled};
+                                              grn_field_table_tiled512, grn_field_table_tiled,
+                                              grn_field_table_exact};
   for (auto fn : tables) {
     int n = 0;
     const GrnKernelEntry *t = fn(&n);
@@ -208,6 +211,20 @@ bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int 
 // comes first: the same wavefronts, less serial work per Euler step.
 std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int path) {
   std::vector<KernelChoice> out;
+  if (path == GRN_PATH_EXACT) {  // the FP64 kernel: state in device memory, any grid
+    (void)n_tgt;
+    (void)guard;
+    for (const GrnKernelEntry *k : c.kernels) {
+      if (k->wc != -2 || G < k->glo || G > k->ghi) continue;
+      KernelChoice kc;
+      kc.k = k;
+      kc.block = k->ntmax;
+      kc.lds = 0;
+      kc.slot_floats = 4L * G * W * H;  // two [G][H][W] double plane sets
+      out.push_back(kc);
+    }
+    return out;
+  }
   for (int pass = 0; pass < 2 && path != GRN_PATH_TILED; ++pass) {  // exact width first, then any width
     const size_t pass_begin = out.size();
     for (const GrnKernelEntry *k : c.kernels) {
@@ -308,6 +325,7 @@ struct BatchOptions {
 struct Plan {
   int n = 0;
   std::vector<Item> items;  // input order
+  std::vector<GrnProgram64> p64;  // [n] when a model runs on the FP64 kernel (else empty)
   std::vector<int> idx;
   std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
   struct Launch {
@@ -336,6 +354,19 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
     if (ds < 0 || ds >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "model %d: unknown dataset %d", i, ds);
     items[i].dataset = ds;
   }
+  // The path each model takes: GRN_PATH_AUTO sends models of datasets with
+  // REF fitness and an early-stop threshold to the FP64 kernel, whose stop
+  // step and penalty are the reference's (the FP32 kernels' rounding of the
+  // convergence measure can move both); everything else to the FP32 kernels.
+  auto model_path = [&](int ds) {
+    if (path != GRN_PATH_AUTO) return path;
+    const grn_sim_params &sp = c->datasets[ds].params;
+    return (sp.fitness_kind == GRN_FITNESS_REF && sp.min_conc_change >= 0) ? (int)GRN_PATH_EXACT : path;
+  };
+  bool any64 = false;
+  for (int i = 0; i < n; ++i) any64 |= model_path(items[i].dataset) == GRN_PATH_EXACT;
+  P->p64.clear();
+  if (any64) P->p64.resize(n);
   // Compile every model (independent work: host threads for large batches).
   std::vector<int> crc(n, 0);
   std::vector<std::string> cerr(n);
@@ -350,7 +381,10 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
       }
       const double dt = (dt_override && dt_override[i] > 0) ? dt_override[i] : D.params.dt;
       const int n_morph = D.params.n_input_morphogens + D.params.n_target_morphogens;
-      crc[i] = grn::compile_model(m, n_morph, items[i].dataset, (float)dt, &items[i].prog, &items[i].info, &cerr[i]);
+      GrnProgram64 *p64 = model_path(items[i].dataset) == GRN_PATH_EXACT ? &P->p64[i] : nullptr;
+      crc[i] = grn::compile_model(m, n_morph, items[i].dataset, (float)dt, &items[i].prog, &items[i].info, &cerr[i],
+                                  p64, dt);
+      if (!crc[i]) crc[i] = grn::validate_program(items[i].prog, (int)c->datasets.size(), &cerr[i]);
     }
   };
   const int n_threads = n >= 1024 ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
@@ -372,7 +406,8 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
     if (opt.force) {
       items[i].kc = *opt.force;
     } else {
-      rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, path, &items[i].kc);
+      rc = choose_kernel(*c, W, H, items[i].prog.G, D.params.n_target_morphogens, !D.nonneg, model_path(ds),
+                         &items[i].kc);
       if (rc) return fail(rc, "model %d: %s", i, g_last_error.c_str());
     }
     items[i].cost = (double)W * H * D.params.num_sim_steps * D.n_exp * items[i].info.flops_per_cell_step;
@@ -454,6 +489,7 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
   const int n = P.n;
   int rc;
   if ((rc = B.progs.reserve(sizeof(GrnProgram) * std::max(n, 1)))) return rc;
+  if (!P.p64.empty() && (rc = B.progs64.reserve(sizeof(GrnProgram64) * n))) return rc;
   if ((rc = B.order.reserve(sizeof(int) * std::max(n, 1)))) return rc;
   if ((rc = B.maxerr.reserve(sizeof(double) * std::max(n, 1)))) return rc;
   if ((rc = B.out.reserve(sizeof(GrnModelOut) * std::max(n, 1)))) return rc;
@@ -476,6 +512,8 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
   }
   if (n > 0) {
     HIP_TRY(hipMemcpyAsync(B.progs.p, c->h_progs, sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    if (!P.p64.empty())
+      HIP_TRY(hipMemcpyAsync(B.progs64.p, P.p64.data(), sizeof(GrnProgram64) * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(B.order.p, P.idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(B.maxerr.p, maxerr->data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   }
@@ -502,6 +540,7 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
     l.tile_sy = kc.tile.sy;
     l.tile_halo = kc.tile.halo;
     l.progs = (const GrnProgram *)B.progs.p;
+    l.progs64 = P.p64.empty() ? nullptr : (const GrnProgram64 *)B.progs64.p;
     l.queue = &((GrnBatch *)B.batch.p)->q[gi];
     l.datasets = (const GrnDataset *)c->d_datasets.p;
     l.max_err = (const double *)B.maxerr.p;
@@ -541,7 +580,8 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
   };
   if (c->launch_mode == GRN_LAUNCH_GRAPH) {
     // the layout: everything a captured launch bakes in
-    std::vector<long> key = {(long)B.batch.p, (long)B.out.p, (long)B.progs.p, (long)B.order.p,
+    std::vector<long> key = {(long)B.batch.p, (long)B.out.p, (long)B.progs.p,
+                             (long)(P.p64.empty() ? nullptr : B.progs64.p), (long)B.order.p,
                              (long)B.maxerr.p, (long)c->d_datasets.p, (long)B.scratch.p,
                              (long)(fields ? fields->p : nullptr), field_stride, field_exp, P.step_cap};
     for (const Plan::Launch &x : P.launches) {
@@ -680,6 +720,18 @@ struct Staged {
   std::vector<double> maxerr;  // a source of the uploads
 };
 
+// Live staged batches.  grn_destroy frees the ones its context still owns,
+// and the staged-batch calls accept only handles found here, so a handle
+// used after its context (or itself) was destroyed is rejected instead of
+// reading freed memory.
+std::mutex g_staged_mu;
+std::set<Staged *> g_staged;
+
+bool staged_live(Staged *S) {
+  std::lock_guard<std::mutex> lk(g_staged_mu);
+  return g_staged.count(S) != 0;
+}
+
 }  // namespace
 
 namespace grn {
@@ -748,6 +800,19 @@ void grn_destroy(void *ctx) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamSynchronize(c->stream2);
+  std::vector<Staged *> mine;
+  {
+    std::lock_guard<std::mutex> lk(g_staged_mu);
+    for (auto it = g_staged.begin(); it != g_staged.end();) {
+      if ((*it)->c == c) {
+        mine.push_back(*it);
+        it = g_staged.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (Staged *S : mine) delete S;
   delete c;
 }
 
@@ -792,6 +857,8 @@ int grn_add_dataset(void *ctx, const grn_sim_params *p, int n_exp, const double 
     g.nonneg = h.nonneg ? 1 : 0;
     g.min_conc = (float)h.params.min_conc;
     g.min_conc_change = (float)h.params.min_conc_change;
+    g.min_conc_d = h.params.min_conc;
+    g.min_conc_change_d = h.params.min_conc_change;
     g.thr = h.params.dist_error_threshold;
     g.init = (const double *)h.d_init;
     g.target = (const double *)h.d_target;
@@ -805,6 +872,46 @@ int grn_add_dataset(void *ctx, const grn_sim_params *p, int n_exp, const double 
   return 0;
 }
 
+int grn_decode_rgb(int width, int height, const unsigned int *rgb, int n_planes, double *out) {
+  if (width < 1 || height < 1 || n_planes < 0 || n_planes > 3 || (n_planes && (!rgb || !out)))
+    return fail(GRN_E_INVALID, "invalid image arguments (%dx%d, %d planes)", width, height, n_planes);
+  // SimState::loadMorphologyImage (simstate.cpp:71-90): Qt's y axis grows
+  // down, so scan line j is row height - 1 - j of the planes.
+  const size_t plane = (size_t)width * height;
+  for (int j = 0; j < height; ++j) {
+    const unsigned int *line = rgb + (size_t)j * width;
+    double *row = out + (size_t)(height - 1 - j) * width;
+    for (int k = 0; k < n_planes; ++k) {
+      const int shift = 16 - 8 * k;
+      double *dst = row + k * plane;
+      for (int i = 0; i < width; ++i) dst[i] = (double)((line[i] >> shift) & 0xffu) / 255.0;
+    }
+  }
+  return 0;
+}
+
+int grn_add_dataset_rgb(void *ctx, const grn_sim_params *p, int n_exp, const unsigned int *init_images,
+                        const unsigned int *target_images, int *dataset_id) {
+  if (!ctx || !p || !dataset_id || n_exp < 1) return fail(GRN_E_INVALID, "invalid arguments");
+  if (p->n_input_morphogens < 0 || p->n_input_morphogens > 3 || p->n_target_morphogens < 0 ||
+      p->n_target_morphogens > 3)
+    return fail(GRN_E_INVALID, "an RGB image holds at most 3 morphogen planes");
+  if (p->dim_x < 1 || p->dim_y < 1) return fail(GRN_E_INVALID, "invalid image size");
+  const size_t N = (size_t)p->dim_x * p->dim_y;
+  const int n_in = p->n_input_morphogens, n_tgt = p->n_target_morphogens;
+  if ((n_in && !init_images) || (n_tgt && !target_images)) return fail(GRN_E_INVALID, "missing images");
+  std::vector<double> init((size_t)n_exp * n_in * N), target((size_t)n_exp * n_tgt * N);
+  for (int e = 0; e < n_exp; ++e) {
+    int rc;
+    if (n_in && (rc = grn_decode_rgb(p->dim_x, p->dim_y, init_images + e * N, n_in, init.data() + e * n_in * N)))
+      return rc;
+    if (n_tgt &&
+        (rc = grn_decode_rgb(p->dim_x, p->dim_y, target_images + e * N, n_tgt, target.data() + e * n_tgt * N)))
+      return rc;
+  }
+  return grn_add_dataset(ctx, p, n_exp, init.data(), target.data(), dataset_id);
+}
+
 int grn_evaluate(void *ctx, int dataset_id, const grn_model *model, double max_error, grn_result *out) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(GRN_E_INVALID, "null context");
@@ -816,7 +923,7 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids, const gr
                        const double *max_errors, const double *dt_override, int path, grn_result *out) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(GRN_E_INVALID, "null context");
-  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_EXACT) return fail(GRN_E_INVALID, "unknown path %d", path);
   HIP_TRY(hipSetDevice(c->device));
   return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, path, out, nullptr, 0, 0, nullptr);
 }
@@ -826,7 +933,7 @@ int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_m
   Ctx *c = as_ctx(ctx);
   if (!c || !staged) return fail(GRN_E_INVALID, "null argument");
   *staged = nullptr;
-  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_EXACT) return fail(GRN_E_INVALID, "unknown path %d", path);
   if (n_models < 1 || !models) return fail(GRN_E_INVALID, "a staged batch needs at least one model");
   HIP_TRY(hipSetDevice(c->device));
   std::unique_ptr<Staged> S(new Staged);
@@ -836,6 +943,10 @@ int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_m
   const hipError_t e = hipStreamSynchronize(c->stream);  // the uploads read S's and the context's host arrays
   if (rc) return rc;
   if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+  {
+    std::lock_guard<std::mutex> lk(g_staged_mu);
+    g_staged.insert(S.get());
+  }
   *staged = S.release();
   return 0;
 }
@@ -844,6 +955,7 @@ int grn_run_staged(void *ctx, void *staged, grn_result *out) {
   Ctx *c = as_ctx(ctx);
   Staged *S = static_cast<Staged *>(staged);
   if (!c || !S || !out) return fail(GRN_E_INVALID, "null argument");
+  if (!staged_live(S)) return fail(GRN_E_INVALID, "the staged batch was destroyed");
   if (S->c != c) return fail(GRN_E_INVALID, "the batch was staged on another context");
   HIP_TRY(hipSetDevice(c->device));
   const auto t_start = std::chrono::steady_clock::now();
@@ -859,6 +971,10 @@ int grn_run_staged(void *ctx, void *staged, grn_result *out) {
 void grn_staged_destroy(void *staged) {
   Staged *S = static_cast<Staged *>(staged);
   if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(g_staged_mu);
+    if (!g_staged.erase(S)) return;  // already freed (by itself or by grn_destroy of its context)
+  }
   (void)hipSetDevice(S->c->device);
   (void)hipStreamSynchronize(S->c->stream);
   delete S;
@@ -868,7 +984,7 @@ int grn_simulate_fields(void *ctx, int dataset_id, const grn_model *model, doubl
                         int path, double *fields, long capacity, int *n_products) {
   Ctx *c = as_ctx(ctx);
   if (!c || !fields) return fail(GRN_E_INVALID, "null argument");
-  if (path < GRN_PATH_AUTO || path > GRN_PATH_TILED) return fail(GRN_E_INVALID, "unknown path %d", path);
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_EXACT) return fail(GRN_E_INVALID, "unknown path %d", path);
   if (dataset_id < 0 || dataset_id >= (int)c->datasets.size()) return fail(GRN_E_INVALID, "unknown dataset");
   const grn_sim_params &p = c->datasets[dataset_id].params;
   if (experiment < 0 || experiment >= c->datasets[dataset_id].n_exp) return fail(GRN_E_INVALID, "bad experiment");

@@ -7,7 +7,7 @@
 #include "grn_program.h"
 
 struct GrnKernelEntry {
-  int wc;     // compile-time grid width, 0 = any width, -1 = halo-tiled kernel
+  int wc;     // compile-time grid width, 0 = any width, -1 = halo-tiled kernel, -2 = FP64 kernel
   int cpt;    // cells per thread (vertical strip length)
   int ntmax;  // workgroup size bound
   int glo;    // smallest gene count handled
@@ -27,6 +27,8 @@ extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wanyguard(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
+// The FP64 kernel (grn_exact.hip): wc = -2.
+extern "C" const GrnKernelEntry *grn_field_table_exact(int *n);
 
 // Zeroes out[0 .. batch->n_total) on stream (grn_probe.hip).
 hipError_t grn_launch_clear_results(GrnBatch *batch, GrnModelOut *out, hipStream_t stream);

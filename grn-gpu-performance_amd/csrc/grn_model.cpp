@@ -182,7 +182,7 @@ bool Model::parse(const std::string &text, Model *out, std::string *err) {
 // ---- compiler ---------------------------------------------------------------
 
 int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnProgram *out,
-                  ProgramInfo *info, std::string *err) {
+                  ProgramInfo *info, std::string *err, GrnProgram64 *out64, double dt64) {
   const std::vector<int> labels = m.labels_in_use(n_morphogens);
   const int G = (int)labels.size();
   if (G > GRN_PMAX_GENES) {
@@ -191,6 +191,10 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
     return GRN_E_CAPACITY;
   }
   std::memset(out, 0, sizeof(*out));
+  if (out64) {
+    std::memset(out64, 0, sizeof(*out64));
+    out64->dt = dt64;
+  }
   out->G = G;
   out->dataset = dataset;
   out->dt = dt;
@@ -212,6 +216,7 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
     gp.deg = (float)g->deg;
     gp.dif = g->dif > 0 ? (float)g->dif : 0.f;  // only dif > 0 diffuses (modelsimulator.cpp:106)
     inf.D += g->dif > 0;
+    if (out64) out64->gene[i] = GrnGene64{g->lim, g->deg, g->dif > 0 ? g->dif : 0.0};
 
     // Links into this gene in model order, split OR / AND; h = 0 unless one
     // of them activates (modelsimulator.cpp:114-138).
@@ -247,6 +252,7 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
         const bool act = l.hill_coef >= 0;
         const double n = act ? l.hill_coef : -l.hill_coef;
         const double nlk = n == 0 ? 0.0 : n * std::log2(l.dis_const);
+        if (out64) out64->link[nl] = GrnLink64{act ? -n : n, act ? nlk : -nlk};
         GrnLink &gl = out->link[nl++];
         gl.r = index_of(l.regulator);
         gl.a = (float)(act ? -n : n);
@@ -281,6 +287,34 @@ int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnPr
   inf.L = nl;
   inf.flops_per_cell_step = 5.0 * inf.L + (inf.L - inf.G_h) + 8.0 * inf.G + 7.0 * inf.D;
   if (info) *info = inf;
+  return 0;
+}
+
+int validate_program(const GrnProgram &p, int n_datasets, std::string *err) {
+  auto bad = [&](const std::string &what) {
+    if (err) *err = "compiled program failed validation: " + what;
+    return GRN_E_MODEL;
+  };
+  if (p.G < 1 || p.G > GRN_PMAX_GENES) return bad("gene count " + std::to_string(p.G));
+  if (p.n_links < 0 || p.n_links > GRN_PMAX_LINKS) return bad("link count " + std::to_string(p.n_links));
+  if (p.dataset < 0 || p.dataset >= n_datasets) return bad("dataset " + std::to_string(p.dataset));
+  if (!std::isfinite(p.dt)) return bad("time step");
+  const int all = (int)((1u << p.G) - 1u);
+  if ((p.reg_mask | p.hill_mask | p.dif_mask) & ~all) return bad("gene masks beyond the genes in use");
+  int hill = 0, dif = 0, reg = 0;
+  for (int g = 0; g < p.G; ++g) {
+    const GrnGene &q = p.gene[g];
+    if (q.n_or < 0 || q.n_and < 0 || q.link_begin < 0 || q.link_begin + q.n_or + q.n_and > p.n_links)
+      return bad("link range of gene " + std::to_string(g));
+    if (q.n_or + q.n_and > 0) hill |= 1 << g;
+    if (q.dif != 0.f) dif |= 1 << g;
+  }
+  for (int l = 0; l < p.n_links; ++l) {
+    const int r = p.link[l].r;
+    if (r < 0 || r >= p.G) return bad("regulator index of link " + std::to_string(l));
+    reg |= 1 << r;
+  }
+  if (hill != p.hill_mask || dif != p.dif_mask || reg != p.reg_mask) return bad("gene masks");
   return 0;
 }
 

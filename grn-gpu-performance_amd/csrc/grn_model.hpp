@@ -65,7 +65,18 @@ struct ProgramInfo {
 // and ModelSimulator::loadModel (Src/Simulator/modelsimulator.cpp:71-203),
 // compiled into exponent-form Hill links (grn_program.h).  Returns 0 or a
 // grn_status code with *err set.
+// out64 (optional) receives the same program's constants in double
+// precision for the FP64 kernel, with dt64 as its time step.
 int compile_model(const Model &m, int n_morphogens, int dataset, float dt, GrnProgram *out,
-                  ProgramInfo *info, std::string *err);
+                  ProgramInfo *info, std::string *err, GrnProgram64 *out64 = nullptr, double dt64 = 0.0);
+
+// Structural check of a compiled program before it is uploaded (the host
+// side of the reference's debug-build GPU_ASSERT bounds checks,
+// Src/GPUSimulator/cerrorhandle.h:24-28): gene count within the kernels'
+// capacity, every gene's link range inside the stored links, every
+// regulator index a gene in use, masks consistent with the genes, the
+// dataset index below n_datasets and a finite time step.  Returns 0 or
+// GRN_E_MODEL with *err naming the first violation.
+int validate_program(const GrnProgram &p, int n_datasets, std::string *err);
 
 }  // namespace grn

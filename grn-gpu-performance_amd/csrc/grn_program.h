@@ -56,6 +56,26 @@ struct GrnProgram {
   GrnLink link[GRN_PMAX_LINKS];
 };
 
+// Double-precision constants of a compiled model for the FP64 kernel
+// (grn_exact.hip), in the same gene and link order as its GrnProgram (which
+// still provides G, the link ranges, the regulator indices and the masks):
+// the same exponent-form Hill links, t = a*log2(x) + b, without rounding a
+// and b to float.
+struct GrnLink64 {
+  double a, b;
+};
+
+struct GrnGene64 {
+  double lim, deg, dif;
+};
+
+struct GrnProgram64 {
+  double dt;
+  double pad;
+  GrnGene64 gene[GRN_PMAX_GENES];
+  GrnLink64 link[GRN_PMAX_LINKS];
+};
+
 struct GrnDataset {
   int W, H;
   int n_in, n_tgt;
@@ -64,6 +84,8 @@ struct GrnDataset {
   int nonneg;     // inputs >= 0 and min_conc >= 0: the diffusion guard is moot
   float min_conc;
   float min_conc_change;
+  double min_conc_d;         // the same two thresholds as given (double)
+  double min_conc_change_d;
   double thr;     // distErrorThreshold
   const double *init;    // [n_exp][n_in][H][W]
   const double *target;  // [n_exp][n_tgt][H][W]
@@ -99,6 +121,7 @@ struct GrnBatch {
 
 struct GrnLaunch {
   const GrnProgram *progs;  // [n_total]
+  const GrnProgram64 *progs64;  // [n_total] (FP64 kernel only; entries of other models unset)
   GrnQueue *queue;          // this launch's queue; the work order comes as a kernel argument
   const GrnDataset *datasets;
   const double *max_err;    // [n]

@@ -379,6 +379,7 @@ static void parallel_for(int n, F f) {
 }
 
 Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {
+  last_generation_ = p.n_generations > 0 ? p.n_generations : -1;
   if (p_.island_size % 2) p_.island_size++;  // searchalgodetcrowd.cpp:33-35
   n_morph_ = p_.n_morphogens;
   for (int i = 0; i <= p_.n_islands; ++i) rngs_.emplace_back(p_.seed * 0x100000001B3ULL + 0x9E37u * (uint64_t)(i + 1));
@@ -733,11 +734,19 @@ void Search::update_stats() {
     }
   s.mean_fit = n ? sum / n : 0;
   s.pareto_size = (int)pareto_.size();
+  s.last_generation = last_generation_;
+  s.best_perfect_complexity = best_perfect_;
+  // the generation just evaluated is generation_ - 1; the reference goes on
+  // while its mean generation is below lastGeneration (search.cpp:158)
+  s.done = (last_generation_ >= 0 && generation_ - 1 >= last_generation_) ? 1 : 0;
   stats_ = s;
 }
 
 void Search::advance(const double *errors) {
-  for (size_t i = 0; i < pending_.size(); ++i) pending_[i]->error = errors[i];
+  for (size_t i = 0; i < pending_.size(); ++i) {
+    pending_[i]->error = errors[i];
+    pending_[i]->generation = generation_;
+  }
   const int n_evaluated = (int)pending_.size();
   // the Pareto front sees the initial population and every selected child
   // (Search::addNewIndividual, search.cpp:323-326)
@@ -747,6 +756,15 @@ void Search::advance(const double *errors) {
   else
     for (int is = 0; is < p_.n_islands; ++is) choose_next_generation(is, &winners);
   update_pareto(winners);
+  // Early end once a perfect model is found (search.cpp:157-167): a new least
+  // complex error-0 model at the head of the front moves the last generation
+  // to 250 past the current one (never beyond nGenerations).
+  if (!pareto_.empty() && pareto_.front().error == 0 &&
+      (best_perfect_ < 0 || pareto_.front().complexity < best_perfect_)) {
+    best_perfect_ = pareto_.front().complexity;
+    const int cut = 250 + generation_;
+    last_generation_ = p_.n_generations > 0 ? std::min(cut, p_.n_generations) : cut;
+  }
   ++generation_;
   update_stats();
   stats_.n_evaluated = n_evaluated;
@@ -1178,6 +1196,46 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   sc.breed_ms = ms_since(t3);
   if (screen) *screen = sc;
   return 0;
+}
+
+static int write_model(const grn::Model &m, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                       int link_cap, int *n_links) {
+  *n_genes = (int)m.genes.size();
+  *n_links = (int)m.links.size();
+  if (*n_genes > gene_cap || *n_links > link_cap || (*n_genes && !genes) || (*n_links && !links))
+    return grn::set_last_error(GRN_E_CAPACITY, "model has %d genes and %d links", *n_genes, *n_links);
+  for (int i = 0; i < *n_genes; ++i) {
+    const grn::Gene &g = m.genes[i];
+    genes[i] = grn_gene{g.label, g.init, g.lim, g.deg, g.dif};
+  }
+  for (int i = 0; i < *n_links; ++i) {
+    const grn::Link &l = m.links[i];
+    links[i] = grn_link{l.regulator, l.regulated, l.dis_const, l.hill_coef, l.is_and ? 1 : 0};
+  }
+  return 0;
+}
+
+int grn_search_pareto(void *search, grn_pareto_entry *entries, int cap, int *n) {
+  SearchHandle *h = as_search(search);
+  if (!h || !n || cap < 0 || (cap && !entries)) return grn::set_last_error(GRN_E_INVALID, "invalid arguments");
+  const std::vector<grn::Individual> &front = h->s->pareto();
+  *n = (int)front.size();
+  for (int i = 0; i < *n && i < cap; ++i) {
+    const grn::Individual &x = front[i];
+    entries[i] = grn_pareto_entry{x.error, x.complexity, x.generation, (int)(x.id >> 40),
+                                  (int)x.model.genes.size(), (int)x.model.links.size(), 0};
+  }
+  return 0;
+}
+
+int grn_search_pareto_model(void *search, int i, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                            int link_cap, int *n_links) {
+  SearchHandle *h = as_search(search);
+  if (!h || !n_genes || !n_links) return grn::set_last_error(GRN_E_INVALID, "null argument");
+  const std::vector<grn::Individual> &front = h->s->pareto();
+  if (i < 0 || i >= (int)front.size())
+    return grn::set_last_error(GRN_E_INVALID, "front member %d of %zu", i, front.size());
+  return write_model(front[i].model, genes, gene_cap, n_genes, links, link_cap, n_links);
 }
 
 int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links, int link_cap,

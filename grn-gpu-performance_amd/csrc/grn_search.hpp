@@ -57,7 +57,8 @@ struct Individual {
   int complexity = 0;          // Model::calcComplexityInUse
   int dataset = 0;
   double dt = 0;               // per-model time step (0: the dataset's)
-  uint64_t id = 0;
+  uint64_t id = 0;             // (island bred on << 40) | serial
+  int generation = -1;         // generation whose evaluation gave its error
   // ln of the simulated raw error where the caller recorded it (NaN if not):
   // the scale a surrogate predicts on, for comparing a child with its parent
   double log_raw = std::numeric_limits<double>::quiet_NaN();
@@ -188,6 +189,9 @@ class Search {
   std::vector<Individual *> pending_;
   std::vector<Individual> pareto_;
   grn_generation_stats stats_{};
+  // Search::calcParalEvolution's end of the run (search.cpp:128-167)
+  int last_generation_ = -1;   // -1: no limit
+  int best_perfect_ = -1;      // complexity of the least complex error-0 model on the front
 };
 
 }  // namespace grn

@@ -67,8 +67,8 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   f.ubase = f.y0 * (TW + 2) + f.x;
   const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt;
   const int S = (a.step_cap > 0 && a.step_cap < D->steps) ? a.step_cap : D->steps;
-  const float mcc = D->min_conc_change;
-  const bool early = mcc >= 0.f;
+  const double mcc = D->min_conc_change_d;
+  const bool early = mcc >= 0.0;
   const double max_err = a.max_err[pi];
   const int plane_floats = G * f.plane;
 
@@ -95,7 +95,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
 
     int s = 0;
     float chg = 1.0f;
-    while (s < S && (!early || chg > mcc)) {
+    while (s < S && (!early || (double)chg > mcc)) {
       const int Tb = early ? 1 : (S - s < T ? S - s : T);
       const bool track = early || s + Tb == S;
       float cblk = 0.f;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
       e_err = tot / ((double)N * n_tgt);
     } else {
       e_err = tot / N;
-      if (chg_acc > (double)mcc) e_err += chg_acc - (double)mcc;  // evaluatormorphogens.cpp:62-63
+      if (chg_acc > mcc) e_err += chg_acc - mcc;  // evaluatormorphogens.cpp:62-63
     }
     err += e_err / n_exp;
     if (a.fields && e == a.field_exp) {

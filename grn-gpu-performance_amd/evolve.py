@@ -155,15 +155,16 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         if progress and gen % 100 == 0:
             progress(f"[evolve] {'screened' if screened else 'plain'} gen {gen} best {g['min_fit']:.4g} "
                      f"t {run.wall[-1]:.1f}s")
-        if g["min_fit"] <= threshold:
+        if g["min_fit"] <= threshold or g["done"]:  # target reached, or the search's own end (search.cpp:157-167)
             break
-    # the best model re-simulated: the error the search reports is the simulator's
+    # the best model re-simulated by the FP64 kernel (the reference's double
+    # precision throughout): the reported error must agree to 1e-5
     best, err = search.best()
     dmax = float(np.max(best.genes["dif"])) if len(best.genes) else 0.0
     dt = min(prob.search["dt_scale"] / dmax, prob.search["dt_cap"]) if dmax > 0 else prob.search["dt_cap"]
-    check = ctx.evaluate_batch([best], datasets=ds, dt=[dt])["fitness"][0]
-    run.verified = bool(check == err)
-    run.best_model = best
+    exact = float(ctx.evaluate_batch([best], datasets=ds, dt=[dt], path=grn.PATH_EXACT)["fitness"][0])
+    run.verified = bool(abs(exact - err) <= 1e-5 * abs(exact))
+    run.best_model, run.best_error, run.best_exact, run.best_dt = best, err, exact, dt
     if sur is not None:
         sur.close()
     search.close()

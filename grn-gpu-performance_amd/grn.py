@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -24,7 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libgrn_hip.so")
 
 FITNESS_REF = 0
 FITNESS_MSE = 1
-PATH_AUTO, PATH_LDS, PATH_TILED = 0, 1, 2
+PATH_AUTO, PATH_LDS, PATH_TILED, PATH_EXACT = 0, 1, 2, 3
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 MAX_GENES = 16
 MAX_LINKS = 96
@@ -38,13 +39,14 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
-    "grn_add_dataset", "grn_evaluate", "grn_evaluate_batch", "grn_stage_batch", "grn_run_staged",
+    "grn_add_dataset", "grn_decode_rgb", "grn_add_dataset_rgb", "grn_evaluate", "grn_evaluate_batch", "grn_stage_batch", "grn_run_staged",
     "grn_staged_destroy", "grn_simulate_fields",
     "grn_get_stats", "grn_test_performance", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
     "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
     "grn_copy_probe",
     "grn_search_default_params", "grn_search_create", "grn_search_destroy", "grn_search_pending",
     "grn_search_advance", "grn_search_run_generation", "grn_search_best",
+    "grn_search_pareto", "grn_search_pareto_model",
     "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
     "grn_search_set_screen_fraction", "grn_search_set_screen_policy", "grn_search_record_raw",
     "grn_search_run_generation_screened", "grn_search_run_generation_coarse",
@@ -111,7 +113,7 @@ class _SearchParams(ctypes.Structure):
                 ("dif_hi", ctypes.c_double), ("k_lo", ctypes.c_double), ("k_hi", ctypes.c_double),
                 ("n_lo", ctypes.c_double), ("n_hi", ctypes.c_double),
                 ("dt_rule", ctypes.c_int), ("dt_scale", ctypes.c_double), ("dt_cap", ctypes.c_double),
-                ("seed", ctypes.c_ulonglong)]
+                ("seed", ctypes.c_ulonglong), ("n_generations", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class _GenerationStats(ctypes.Structure):
@@ -119,6 +121,13 @@ class _GenerationStats(ctypes.Structure):
                 ("min_fit", ctypes.c_double), ("mean_fit", ctypes.c_double),
                 ("max_fit", ctypes.c_double), ("best_complexity", ctypes.c_int),
                 ("best_island", ctypes.c_int), ("pareto_size", ctypes.c_int),
+                ("done", ctypes.c_int), ("last_generation", ctypes.c_int),
+                ("best_perfect_complexity", ctypes.c_int)]
+
+
+class _ParetoEntry(ctypes.Structure):
+    _fields_ = [("error", ctypes.c_double), ("complexity", ctypes.c_int), ("generation", ctypes.c_int),
+                ("island", ctypes.c_int), ("n_genes", ctypes.c_int), ("n_links", ctypes.c_int),
                 ("reserved", ctypes.c_int)]
 
 
@@ -201,6 +210,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_destroy.argtypes = [vp]
     lib.grn_destroy.restype = None
     lib.grn_add_dataset.argtypes = [vp, ctypes.POINTER(_SimParams), ctypes.c_int, vp, vp, ip]
+    lib.grn_decode_rgb.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp]
+    lib.grn_add_dataset_rgb.argtypes = [vp, ctypes.POINTER(_SimParams), ctypes.c_int, vp, vp, ip]
     lib.grn_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
                                  ctypes.POINTER(_Result)]
     lib.grn_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp]
@@ -233,6 +244,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_advance.argtypes = [vp, vp, ctypes.POINTER(_GenerationStats)]
     lib.grn_search_run_generation.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_GenerationStats)]
     lib.grn_search_best.argtypes = [vp, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip, dp]
+    lib.grn_search_pareto.argtypes = [vp, vp, ctypes.c_int, ip]
+    lib.grn_search_pareto_model.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ip, vp, ctypes.c_int, ip]
     fp = ctypes.POINTER(ctypes.c_float)
     lib.grn_search_set_oversample.argtypes = [vp, ctypes.c_int]
     lib.grn_search_candidates.argtypes = [vp, ip, ctypes.POINTER(vp)]
@@ -330,6 +343,22 @@ class Model:
         return g.value, l.value, f.value
 
 
+def decode_rgb(images: np.ndarray, n_planes: int) -> np.ndarray:
+    """SimState::loadMorphologyImage (Src/Simulator/simstate.cpp:71-90) of
+    images[n][H][W] (uint32 0xAARRGGBB, scan lines top to bottom) ->
+    planes [n][n_planes][H][W] (float64): plane k is byte 2 - k / 255, image
+    line j is plane row H - 1 - j."""
+    img = np.ascontiguousarray(images, np.uint32)
+    n, H, W = img.shape
+    out = np.zeros((n, n_planes, H, W), np.float64)
+    lib = load_library()
+    for e in range(n):
+        o = np.zeros((n_planes, H, W), np.float64)
+        _check(lib.grn_decode_rgb(W, H, img[e].ctypes.data, n_planes, o.ctypes.data))
+        out[e] = o
+    return out
+
+
 def ceil_s(n: float, digits: int) -> float:
     """MathAlgo::ceilS (Src/Common/mathalgo.h:55-62)."""
     return load_library().grn_ceil_s(n, digits)
@@ -403,9 +432,12 @@ class Context:
         self._h = h
         self._lib = lib
         self.params: list[SimParams] = []
+        self._staged = weakref.WeakSet()  # staged batches to free before the context
 
     def close(self):
         if self._h:
+            for s in list(self._staged):
+                s.close()
             self._lib.grn_destroy(self._h)
             self._h = None
 
@@ -432,6 +464,23 @@ class Context:
                                          init.ctypes.data if init.size else None,
                                          target.ctypes.data if target.size else None,
                                          ctypes.byref(ds)))
+        self.params.append(params)
+        return ds.value
+
+    def add_dataset_rgb(self, params: SimParams, init_images: np.ndarray, target_images: np.ndarray) -> int:
+        """Experiment images as the reference stores them (uint32 0xAARRGGBB,
+        [n_exp][dim_y][dim_x], scan lines top to bottom): the input planes from
+        init_images, the target planes from target_images (decode_rgb)."""
+        ii = np.ascontiguousarray(init_images, dtype=np.uint32)
+        ti = np.ascontiguousarray(target_images, dtype=np.uint32)
+        n_exp = ii.shape[0] if ii.ndim == 3 else ti.shape[0]
+        for a in (ii, ti):
+            if a.size and a.shape != (n_exp, params.dim_y, params.dim_x):
+                raise ValueError(f"images must be [n_exp][{params.dim_y}][{params.dim_x}]")
+        cp = params._c()
+        ds = ctypes.c_int()
+        _check(self._lib.grn_add_dataset_rgb(self._h, ctypes.byref(cp), n_exp, ii.ctypes.data if ii.size else None,
+                                             ti.ctypes.data if ti.size else None, ctypes.byref(ds)))
         self.params.append(params)
         return ds.value
 
@@ -481,7 +530,9 @@ class Context:
             self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
             None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data,
             path, ctypes.byref(h)))
-        return StagedBatch(self, h.value, n)
+        sb = StagedBatch(self, h.value, n)
+        self._staged.add(sb)
+        return sb
 
     def simulate_fields(self, model: Model, dataset: int = 0, experiment: int = 0,
                         dt: float = 0.0, path: int = PATH_AUTO) -> np.ndarray:
@@ -680,6 +731,26 @@ class Search:
         _check(self._lib.grn_search_run_generation_coarse(self._h, ctx._h, ds.ctypes.data, float(dif_scale), path,
                                                           ctypes.byref(st), ctypes.byref(ms)))
         return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}, ms.value
+
+    def pareto(self) -> list:
+        """The Pareto front on (error, complexity in use) as
+        Search::recalculateParetoFront keeps it (search.cpp:348-377): one dict
+        per member (error, complexity, generation, island, model)."""
+        n = ctypes.c_int()
+        _check(self._lib.grn_search_pareto(self._h, None, 0, ctypes.byref(n)))
+        arr = (_ParetoEntry * max(n.value, 1))()
+        _check(self._lib.grn_search_pareto(self._h, ctypes.addressof(arr), n.value, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            e = arr[i]
+            g = np.zeros(max(e.n_genes, 1), GENE_DTYPE)
+            l = np.zeros(max(e.n_links, 1), LINK_DTYPE)
+            ng, nl = ctypes.c_int(), ctypes.c_int()
+            _check(self._lib.grn_search_pareto_model(self._h, i, g.ctypes.data, len(g), ctypes.byref(ng),
+                                                     l.ctypes.data, len(l), ctypes.byref(nl)))
+            out.append({"error": e.error, "complexity": e.complexity, "generation": e.generation,
+                        "island": e.island, "model": Model(g[:ng.value].copy(), l[:nl.value].copy())})
+        return out
 
     def best(self):
         """(Model, error) of the lowest-error individual (ties: lower complexity)."""

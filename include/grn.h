@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define GRN_ABI_VERSION 2
+#define GRN_ABI_VERSION 3
 
 /* Capacity of the device program (compile-time limits of the kernels). */
 #define GRN_MAX_GENES 16  /* genes in use per model */
@@ -63,9 +63,13 @@ typedef enum {
 
 /* Path selection for grn_evaluate_batch. */
 typedef enum {
-  GRN_PATH_AUTO = 0,  /* tuned per size class */
-  GRN_PATH_LDS = 1,   /* field resident on chip (registers + LDS halo) */
-  GRN_PATH_TILED = 2  /* field in device memory, halo-tiled */
+  GRN_PATH_AUTO = 0,  /* FP64 kernel for datasets with REF fitness and an early-stop
+                         threshold (min_conc_change >= 0), else the FP32 kernels
+                         tuned per size class */
+  GRN_PATH_LDS = 1,   /* FP32, field resident on chip (registers + LDS halo) */
+  GRN_PATH_TILED = 2, /* FP32, field in device memory, halo-tiled */
+  GRN_PATH_EXACT = 3  /* FP64 state and arithmetic throughout, as the reference
+                         (modelsimulator.cpp, cmodeldata.h: double everywhere) */
 } grn_path;
 
 /* The SimParams fields the simulation and fitness read
@@ -154,6 +158,24 @@ void grn_destroy(void *ctx);
 int grn_add_dataset(void *ctx, const grn_sim_params *params, int n_experiments,
                     const double *init, const double *target, int *dataset_id);
 
+/* SimState::loadMorphologyImage (Src/Simulator/simstate.cpp:71-90): decodes
+ * n_planes (<= 3) channels of a width x height image -- packed 0xAARRGGBB
+ * pixels (QRgb), scan lines top to bottom -- into planes
+ * out[n_planes][height][width] (x fastest): plane k is byte 2 - k of the pixel
+ * (red, green, blue) / 255 and image scan line j is plane row
+ * height - 1 - j.  Host only (no device needed). */
+int grn_decode_rgb(int width, int height, const unsigned int *rgb, int n_planes, double *out);
+
+/* grn_add_dataset from the reference's experiment images
+ * (CEvaluatorMorphogens::createGPUExperimentDataset,
+ * cevaluatormorphogens.cpp:113-131, which loads the input and output
+ * morphologies with loadMorphologyImage): init_images[n_experiments][dim_y]
+ * [dim_x] give the n_input_morphogens input planes of each experiment and
+ * target_images the n_target_morphogens target planes (each at most 3).
+ * Images are dim_x x dim_y pixels, decoded as grn_decode_rgb. */
+int grn_add_dataset_rgb(void *ctx, const grn_sim_params *params, int n_experiments,
+                        const unsigned int *init_images, const unsigned int *target_images, int *dataset_id);
+
 /* Plain path: compile, launch one workgroup, synchronize -- one model per
  * launch as the reference does. */
 int grn_evaluate(void *ctx, int dataset_id, const grn_model *model,
@@ -173,7 +195,9 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
  * the same arguments.  For repeated evaluation of one population (timing,
  * the known-answer self-test); no counterpart in the reference, which
  * uploads every batch.  The handle belongs to ctx; grn_staged_destroy frees
- * it (before grn_destroy of its context). */
+ * it, and grn_destroy of the context frees those still alive.  A handle
+ * that was freed either way is rejected by grn_run_staged (GRN_E_INVALID)
+ * and ignored by grn_staged_destroy. */
 int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
                     const double *max_errors, const double *dt_override, int path, void **staged);
 int grn_run_staged(void *ctx, void *staged, grn_result *out);
@@ -322,6 +346,8 @@ typedef struct {
   int dt_rule;              /* 0: the dataset's dt; 1: dt_scale / max dif, at most dt_cap */
   double dt_scale, dt_cap;
   unsigned long long seed;
+  int n_generations;        /* SearchParams.nGenerations: the last generation (<= 0: no limit) */
+  int reserved;
 } grn_search_params;
 
 typedef struct {
@@ -331,8 +357,24 @@ typedef struct {
   int best_complexity;      /* complexity in use of the best individual */
   int best_island;
   int pareto_size;          /* size of the (error, complexity) Pareto front */
-  int reserved;
+  int done;                 /* 1: the search has reached its last generation */
+  int last_generation;      /* n_generations, or 250 generations after the least complex
+                               perfect model found (search.cpp:157-167); -1: no limit */
+  int best_perfect_complexity; /* complexity of the least complex error-0 model, -1: none yet */
 } grn_generation_stats;
+
+/* One member of the Pareto front on (error, complexity in use) that
+ * Search::recalculateParetoFront keeps (search.cpp:348-377): sorted by
+ * error, then complexity; each member less complex than the one before it,
+ * or an exact duplicate (same error and complexity) of it. */
+typedef struct {
+  double error;
+  int complexity;           /* Model::calcComplexityInUse */
+  int generation;           /* generation whose evaluation gave its error (0: initial population) */
+  int island;               /* island it was bred on */
+  int n_genes, n_links;     /* size of its model (grn_search_pareto_model) */
+  int reserved;
+} grn_pareto_entry;
 
 /* The reference's defaults for everything grn_search_params holds. */
 void grn_search_default_params(grn_search_params *p);
@@ -355,6 +397,13 @@ int grn_search_advance(void *search, const double *errors, grn_generation_stats 
 
 /* grn_search_pending + grn_evaluate_batch on ctx + grn_search_advance. */
 int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats);
+
+/* The current Pareto front: writes up to cap entries, *n gets its size. */
+int grn_search_pareto(void *search, grn_pareto_entry *entries, int cap, int *n);
+
+/* The model of front member i (as grn_search_best writes models). */
+int grn_search_pareto_model(void *search, int i, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,
+                            int link_cap, int *n_links);
 
 /* The lowest-error individual over all islands (ties: lower complexity). */
 int grn_search_best(void *search, grn_gene *genes, int gene_cap, int *n_genes, grn_link *links,

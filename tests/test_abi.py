@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     lib = grn.load_library()
-    assert lib.grn_abi_version() == 2
+    assert lib.grn_abi_version() == 3
     assert lib.grn_last_error() is not None
 
 
@@ -126,7 +126,8 @@ def test_struct_layouts_match_header(tmp_path):
                "grn_stats": grn._Stats, "grn_search_params": grn._SearchParams,
                "grn_generation_stats": grn._GenerationStats, "grn_surrogate_params": grn._SurrogateParams,
                "grn_surrogate_stats": grn._SurrogateStats, "grn_screen_stats": grn._ScreenStats,
-               "grn_tune_candidate": grn._TuneCandidate, "grn_tune_class": grn._TuneClass}
+               "grn_tune_candidate": grn._TuneCandidate, "grn_tune_class": grn._TuneClass,
+               "grn_pareto_entry": grn._ParetoEntry}
     dtypes = {"grn_gene": grn.GENE_DTYPE, "grn_link": grn.LINK_DTYPE}
     structs = {k: [f for f, _ in v._fields_] for k, v in mirrors.items()}
     structs.update({k: list(v.names) for k, v in dtypes.items()})
@@ -154,3 +155,20 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_decode_rgb_matches_oracle_and_known_pixels():
+    """grn_decode_rgb (the library's SimState::loadMorphologyImage) against
+    the oracle's restatement and against pixels with known channel bytes."""
+    import grn_oracle as ora
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 2 ** 32, (2, 5, 7), dtype=np.uint64).astype(np.uint32)
+    for n_planes in (1, 2, 3):
+        np.testing.assert_array_equal(grn.decode_rgb(img, n_planes), ora.decode_rgb(img, n_planes))
+    px = np.array([[[0xFF0A1428, 0x00000000], [0x00FFFFFF, 0x12345678]]], np.uint32)
+    out = grn.decode_rgb(px, 3)
+    np.testing.assert_array_equal(out[0, :, 1, 0], [10 / 255, 20 / 255, 40 / 255])  # top line -> last row
+    np.testing.assert_array_equal(out[0, :, 0, 0], [1.0, 1.0, 1.0])
+    np.testing.assert_array_equal(out[0, :, 0, 1], [0x34 / 255, 0x56 / 255, 0x78 / 255])
+    with pytest.raises(grn.GrnError):
+        grn.decode_rgb(px, 4)

@@ -1,13 +1,26 @@
 """Evolution to a fitness target through the batched path, plain and
 surrogate-screened: deterministic for a seed, every reported best error
-re-simulates to the same value, and the best never rises."""
+agrees within 1e-5 with the FP64 kernel and with the FP64 CPU oracle, and
+the best never rises."""
 import numpy as np
 import pytest
 
 import grn
+import grn_oracle as ora
 import evolve
 
 pytestmark = pytest.mark.gpu
+
+
+def check_best_with_oracle(prob, run):
+    """The run's reported best error against the oracle's evaluation of its
+    best model (same time step): within 1e-5 relative."""
+    p = prob.params
+    op = ora.params(p.dim_x, p.dim_y, p.n_input_morphogens, p.n_target_morphogens, p.num_sim_steps, run.best_dt,
+                    fitness_kind=p.fitness_kind)
+    want = ora.evaluate(op, prob.init, prob.target, run.best_model.genes, run.best_model.links)[0]["fitness"]
+    assert abs(run.best_error - want) <= 1e-5 * want, (run.best_error, want)
+    assert abs(run.best_exact - want) <= 1e-10 * want
 
 
 def test_plain_and_screened_evolution_are_verified_and_deterministic():
@@ -23,6 +36,8 @@ def test_plain_and_screened_evolution_are_verified_and_deterministic():
     assert s.verified and all(x >= y for x, y in zip(s.best, s.best[1:]))
     assert s.best[0] == a.best[0]  # the same initial population
     assert all(sc["n_candidates"] in (0, 4 * 512) for sc in s.screen)
+    check_best_with_oracle(prob, a)
+    check_best_with_oracle(prob, s)
     ctx.close()
 
 

@@ -1,13 +1,17 @@
 """Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
 
-Tolerances (FP32 compensated state on the device vs FP64 on the host):
-  * fitness (raw error):   |gpu - cpu| <= 1e-5 * |cpu|
-    except the REF-mode non-convergence penalty (minConcChange crossing,
-    evaluatormorphogens.cpp:62-63), whose input -- the last step's max
-    |dc/dt| -- is an FP32 residual: that component gets 2e-4 relative
+Tolerances (device vs the FP64 oracle on the host):
+  * fitness (raw error and the rounded fitness):  |gpu - cpu| <= 1e-5 * |cpu|
   * final field per cell:  |gpu - cpu| <= 1e-4 * |cpu| + 1e-7
   * ranking of a population by fitness: identical order
-Integer outputs (steps executed, experiments run, genes in use) are exact.
+Integer outputs (steps executed, experiments run, genes in use) are exact, for
+every model.  The FP32 kernels (compensated state) meet these bars with a
+fixed step count; runs with REF fitness and an early-stop threshold -- where
+the step count is an integer decision on the last step's max |dc/dt| and that
+value is the non-convergence penalty -- take the FP64 kernel by default and
+meet them too (it agrees with the oracle to ~1e-12).  The FP32 kernels forced
+onto such runs are checked separately with the looser bars their rounding
+allows (tests named *_fp32_forced).
 """
 import dataclasses
 
@@ -134,24 +138,88 @@ def test_models_with_non_diffusing_genes(ctx, W, H, min_conc):
         check_fields(f, fo)
 
 
+def check_exact_run(p, got, want):
+    """Every model: the same step count, the raw error and the rounded
+    fitness within 1e-5 of the oracle, the same ranking."""
+    assert np.array_equal(got["steps"], want["steps"])
+    assert np.array_equal(got["experiments"], want["experiments"])
+    check_fitness(got["raw_error"], want["raw_error"])
+    check_fitness(got["fitness"], want["fitness"])
+    np.testing.assert_allclose(got["change"], want["change"], rtol=1e-8)
+    assert np.array_equal(np.argsort(got["fitness"], kind="stable"), np.argsort(want["fitness"], kind="stable"))
+
+
 def test_ref_fitness_with_early_stop_penalty_and_rounding(ctx):
+    """The reference's default contract: REF fitness, minConcChange 1e-3,
+    errorPrecision 6 (evaluatormorphogens.cpp:48-76).  The default path runs
+    these models on the FP64 kernel: identical step counts, penalties and
+    rounded fitness for every model."""
     p, init, target, models = population_case(21, 24, 32, 32, 3000, g_range=(3, 7), l_range=(1, 12),
                                               kind=grn.FITNESS_REF, mcc=1e-3, thr=0.05, dt=0.05)
     p.error_precision = 6
     ds, got, want = run_both(ctx, p, init, target, models)
-    # the stop step is an integer decision on a float threshold: FP32 vs FP64
-    # may differ by a step only when the change sits on the threshold
+    check_exact_run(p, got, want)
+    assert np.any(got["steps"] < 3000)                     # some models do stop early
+    assert np.any(want["change"] > p.min_conc_change)      # and some pay the penalty
+    # the agreement is far inside the bar
+    assert np.max(np.abs(got["raw_error"] - want["raw_error"]) / want["raw_error"]) < 1e-9
+
+
+def test_ref_early_stop_fp32_forced(ctx):
+    """The FP32 on-chip kernel forced onto the same kind of run: its last-step
+    max |dc/dt| is an FP32 residual (rounding of lim*h ~1e-7, about 1e-4 of a
+    rate near 1e-3), so the stop may move by a step for models whose change
+    sits on the threshold and the penalty agrees to 2e-4; the log-distance
+    part holds the 1e-5 bar."""
+    p, init, target, models = population_case(21, 24, 32, 32, 3000, g_range=(3, 7), l_range=(1, 12),
+                                              kind=grn.FITNESS_REF, mcc=1e-3, thr=0.05, dt=0.05)
+    ds = ctx.add_dataset(p, init, target)
+    got = ctx.evaluate_batch(models, datasets=ds, path=grn.PATH_LDS)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, n_threads=8)
     assert np.mean(got["steps"] == want["steps"]) >= 0.9
     same = got["steps"] == want["steps"]
-    # split the error into the log-distance part and the penalty part
     pen_w = np.where(want["change"] > p.min_conc_change, want["change"] - p.min_conc_change, 0.0)
     pen_g = np.where(got["change"] > p.min_conc_change, got["change"] - p.min_conc_change, 0.0)
-    dist_w, dist_g = want["raw_error"] - pen_w, got["raw_error"] - pen_g
-    check_fitness(dist_g[same], dist_w[same])
+    check_fitness((got["raw_error"] - pen_g)[same], (want["raw_error"] - pen_w)[same])
     assert np.all(np.abs(pen_g - pen_w)[same] <= 2e-4 * pen_w[same] + 1e-12)
-    assert np.all(np.abs(got["change"] - want["change"])[same] <= 2e-4 * want["change"][same])
-    assert np.any(got["steps"][same] < 3000)  # some models do stop early
-    assert np.any(pen_w[same] > 0)            # and some pay the penalty
+
+
+@pytest.mark.parametrize("W,H,S,kind,mcc", [(32, 32, 400, grn.FITNESS_MSE, -1.0), (20, 12, 300, grn.FITNESS_REF, 2e-3),
+                                            (64, 64, 200, grn.FITNESS_REF, -1.0), (100, 100, 150, grn.FITNESS_MSE, -1.0),
+                                            (128, 96, 120, grn.FITNESS_REF, 5e-3)])
+def test_exact_path_matches_oracle(ctx, W, H, S, kind, mcc):
+    """GRN_PATH_EXACT (FP64 state and arithmetic) on any grid, fixed or
+    early-stopped runs: fitness and every final field to 1e-10 of the oracle."""
+    p, init, target, models = population_case(61 + W + H, 12, W, H, S, n_in=2, g_range=(3, 12), l_range=(1, 30),
+                                              kind=kind, mcc=mcc, thr=0.02, dt=0.05)
+    ds = ctx.add_dataset(p, init, target)
+    got = ctx.evaluate_batch(models, datasets=ds, path=grn.PATH_EXACT)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, n_threads=8)
+    assert np.array_equal(got["steps"], want["steps"])
+    np.testing.assert_allclose(got["raw_error"], want["raw_error"], rtol=1e-10)
+    for m in models[:2]:
+        f = ctx.simulate_fields(m, dataset=ds, path=grn.PATH_EXACT)
+        _, fo = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, want_fields=True)
+        np.testing.assert_allclose(f, fo, rtol=1e-10, atol=1e-13)
+
+
+def test_batch_invariance_across_grids_in_one_class(ctx):
+    """Models of different grids that share a kernel class (and with it the
+    class's workgroup size) give bitwise the fitness they give alone: 32x16
+    and 32x32 on the 32-wide kernel, 40x40, 56x56 and 40x24 on the any-width
+    kernel."""
+    ids, models = [], []
+    for k, (W, H) in enumerate([(32, 16), (32, 32), (40, 40), (56, 56), (40, 24)]):
+        p, init, target, ms = population_case(91 + k, 4, W, H, 120, n_in=2, g_range=(3, 7), l_range=(1, 14))
+        ds = ctx.add_dataset(p, init, target)
+        ids += [ds] * len(ms)
+        models += ms
+    together = ctx.evaluate_batch(models, datasets=ids)
+    assert ctx.stats()["n_launches"] < len({i for i in ids})  # grids do share classes
+    for i, m in enumerate(models):
+        alone = ctx.evaluate_batch([m], datasets=[ids[i]])
+        assert alone["raw_error"][0] == together["raw_error"][i], i
+        assert alone["steps"][0] == together["steps"][i]
 
 
 def test_multiple_experiments_and_max_error_cut(ctx):
@@ -305,15 +373,21 @@ def test_tiled_equals_on_chip_fields(ctx, W, H):
 
 
 def test_tiled_early_stop_and_guard(ctx):
+    """A 100x100 early-stop REF run (the FP64 kernel by default: exact step
+    counts and fitness), the same run forced onto the FP32 halo-tiled kernel
+    (looser bars, as for the on-chip one), and negative inputs on the guarded
+    tiled kernel."""
     W, H = 100, 100
     p, init, target, models = population_case(51, 8, W, H, 1500, g_range=(3, 6), l_range=(1, 10),
                                               kind=grn.FITNESS_REF, mcc=2e-3, thr=0.05, dt=0.05)
     ds, got, want = run_both(ctx, p, init, target, models)
-    assert np.mean(got["steps"] == want["steps"]) >= 0.75
-    same = got["steps"] == want["steps"]
+    check_exact_run(p, got, want)
+    tiled = ctx.evaluate_batch(models, datasets=ds, path=grn.PATH_TILED)
+    assert np.mean(tiled["steps"] == want["steps"]) >= 0.75
+    same = tiled["steps"] == want["steps"]
     pen_w = np.where(want["change"] > p.min_conc_change, want["change"] - p.min_conc_change, 0.0)
-    pen_g = np.where(got["change"] > p.min_conc_change, got["change"] - p.min_conc_change, 0.0)
-    check_fitness((got["raw_error"] - pen_g)[same], (want["raw_error"] - pen_w)[same])
+    pen_g = np.where(tiled["change"] > p.min_conc_change, tiled["change"] - p.min_conc_change, 0.0)
+    check_fitness((tiled["raw_error"] - pen_g)[same], (want["raw_error"] - pen_w)[same])
     # negative inputs route to the guarded tiled kernel
     rng = np.random.default_rng(3)
     q = grn.SimParams(W, H, 2, 1, 100, 0.02, min_conc=-10.0, fitness_kind=grn.FITNESS_MSE)
@@ -356,3 +430,24 @@ def test_model_batch_equals_list(ctx):
     b = grn.ModelBatch(w.models)
     assert np.array_equal(ctx.evaluate_batch(b, datasets=ds), a)
     assert np.array_equal(ctx.evaluate_batch(b, datasets=ds), a)
+
+
+def test_rgb_experiment_images(ctx):
+    """Experiments given as the reference's images (grn_add_dataset_rgb,
+    SimState::loadMorphologyImage) evaluate exactly as the planes the
+    oracle's restatement decodes from them, and match the oracle."""
+    rng = np.random.default_rng(17)
+    W, H = 24, 16
+    init_img = rng.integers(0, 2 ** 32, (2, H, W), dtype=np.uint64).astype(np.uint32)
+    tgt_img = rng.integers(0, 2 ** 32, (2, H, W), dtype=np.uint64).astype(np.uint32)
+    p = grn.SimParams(W, H, 3, 2, 150, 0.05, fitness_kind=grn.FITNESS_REF, dist_error_threshold=0.05)
+    ds_img = ctx.add_dataset_rgb(p, init_img, tgt_img)
+    init, target = ora.decode_rgb(init_img, 3), ora.decode_rgb(tgt_img, 2)
+    ds_f = ctx.add_dataset(p, init, target)
+    models = workloads._population(rng, 12, (5, 9), (2, 20), 5)
+    a = ctx.evaluate_batch(models, datasets=ds_img)
+    b = ctx.evaluate_batch(models, datasets=ds_f)
+    assert np.array_equal(a, b)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, models, n_threads=8)
+    check_fitness(a["raw_error"], want["raw_error"])
+    assert np.all(a["experiments"] == 2)

@@ -336,3 +336,77 @@ def test_coarse_generation_rejects_bad_arguments():
     with pytest.raises(ValueError):
         s.run_generation_coarse(None, [0, 0, 0], 0.25)  # one coarse dataset per island
     s.close()
+
+
+def complexity_in_use(m, n_morph):
+    """Model::calcComplexityInUse (model.cpp:590-611): 1 + (dif > 0) per
+    product in use (modelprod.cpp:39-41), 1 per link between products in use."""
+    used = set(int(x) for x in ora.labels_in_use(m.genes, m.links, n_morph))
+    c = sum(1 + (d > 0) for lab, d in zip(m.genes["label"], m.genes["dif"]) if int(lab) in used)
+    return c + sum(1 for r, t in zip(m.links["regulator"], m.links["regulated"]) if int(r) in used and int(t) in used)
+
+
+def ref_front(front, fresh):
+    """Search::recalculateParetoFront (search.cpp:348-377) on (error,
+    complexity) pairs: sort by error then complexity, keep each entry less
+    complex than the last kept one, and exact duplicates of it."""
+    out = []
+    for e, c in sorted(front + fresh):
+        if not out or c < out[-1][1] or (c == out[-1][1] and e == out[-1][0]):
+            out.append((e, c))
+    return out
+
+
+def coarse_error(m, dataset=0):
+    """fake_error rounded to 0.05: many exact ties, so the front's duplicate
+    rule is exercised."""
+    return round(fake_error(m, dataset) * 20) / 20
+
+
+def test_pareto_front_follows_the_reference_rule():
+    """The front the library keeps equals the reference's rule applied to the
+    initial population and every selected child (a child is selected when
+    its error is at most its parent's, which is its maxError), generation by
+    generation; each member's model reproduces its error and complexity."""
+    n_morph = 3
+    s = grn.Search(n_islands=3, island_size=24, n_morphogens=n_morph, migration_period=4, seed=21)
+    front = []
+    for gen in range(25):
+        models, ds, me, dt = s.pending()
+        errs = np.array([coarse_error(m, d) for m, d in zip(models, ds)])
+        fresh = [(float(e), complexity_in_use(m, n_morph)) for m, e, mx in zip(models, errs, me)
+                 if gen == 0 or e <= mx]
+        front = ref_front(front, fresh)
+        st = s.advance(errs)
+        got = s.pareto()
+        assert [(x["error"], x["complexity"]) for x in got] == front, gen
+        assert st["pareto_size"] == len(front)
+    assert any(a == b for a, b in zip(front, front[1:]))  # duplicates kept
+    for x in got:
+        assert coarse_error(x["model"]) == x["error"]
+        assert complexity_in_use(x["model"], n_morph) == x["complexity"]
+        assert 0 <= x["generation"] < 25 and 0 <= x["island"] < 3
+    assert st["done"] == 0 and st["last_generation"] == -1 and st["best_perfect_complexity"] == -1
+
+
+def test_perfect_model_moves_the_last_generation():
+    """search.cpp:157-167: once the front's head has error 0 with a new
+    lowest complexity, the run ends 250 generations later (never after
+    nGenerations); the search reports done when it gets there."""
+    def perfect_if_small(m, dataset=0):
+        # error 0 for models with at most two links, else the fake error
+        return 0.0 if len(m.links) <= 2 else fake_error(m, dataset)
+
+    s = grn.Search(n_islands=2, island_size=16, n_morphogens=3, seed=5, n_generations=1000)
+    st = s.advance(np.array([perfect_if_small(m) for m in s.pending()[0]]))
+    head = s.pareto()[0]
+    assert head["error"] == 0.0
+    assert st["best_perfect_complexity"] == head["complexity"]
+    assert st["last_generation"] == 250 and st["done"] == 0
+    s2 = grn.Search(n_islands=2, island_size=16, n_morphogens=3, seed=5, n_generations=100)
+    st2 = s2.advance(np.array([perfect_if_small(m) for m in s2.pending()[0]]))
+    assert st2["last_generation"] == 100  # capped by nGenerations
+    # with a limit of 3 generations the search is done after generation 3
+    s3 = grn.Search(n_islands=1, island_size=8, n_morphogens=3, seed=2, n_generations=3)
+    done = [st["done"] for st, _, _ in drive(s3, 5)]
+    assert done == [0, 0, 0, 1, 1]
