@@ -109,6 +109,16 @@ struct BatchBufs {
   DevBuf progs, progs64, order, maxerr, out, batch, scratch;
 };
 
+// Device addresses a batch's launches read and write: a BatchBufs' buffers,
+// or regions of the fixed scratch array of grn_evaluate_batch_noalloc.
+struct DevPtrs {
+  void *progs, *progs64, *order, *maxerr, *out, *batch, *scratch;
+};
+
+DevPtrs ptrs_of(const BatchBufs &B) {
+  return DevPtrs{B.progs.p, B.progs64.p, B.order.p, B.maxerr.p, B.out.p, B.batch.p, B.scratch.p};
+}
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -524,7 +534,7 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
 // Enqueues the batch: result clearing (which also rewinds the work queues),
 // then one persistent kernel per class, eagerly or as a captured graph,
 // bracketed by the context's timing events.
-int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_stride, int field_exp,
+int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long field_stride, int field_exp,
                 bool *replayed) {
   *replayed = false;
   if (P.n == 0) return 0;
@@ -532,19 +542,19 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
   for (size_t gi = 0; gi < P.launches.size(); ++gi) {
     const KernelChoice &kc = P.launches[gi].kc;
     GrnLaunch &l = L[gi];
-    l.scratch = (float *)B.scratch.p + P.launches[gi].scratch_off;
+    l.scratch = (float *)B.scratch + P.launches[gi].scratch_off;
     l.slot_floats = kc.slot_floats;
     l.tile_w = kc.tile.tw;
     l.tile_h = kc.tile.th;
     l.tile_sx = kc.tile.sx;
     l.tile_sy = kc.tile.sy;
     l.tile_halo = kc.tile.halo;
-    l.progs = (const GrnProgram *)B.progs.p;
-    l.progs64 = P.p64.empty() ? nullptr : (const GrnProgram64 *)B.progs64.p;
-    l.queue = &((GrnBatch *)B.batch.p)->q[gi];
+    l.progs = (const GrnProgram *)B.progs;
+    l.progs64 = P.p64.empty() ? nullptr : (const GrnProgram64 *)B.progs64;
+    l.queue = &((GrnBatch *)B.batch)->q[gi];
     l.datasets = (const GrnDataset *)c->d_datasets.p;
-    l.max_err = (const double *)B.maxerr.p;
-    l.out = (GrnModelOut *)B.out.p;
+    l.max_err = (const double *)B.maxerr;
+    l.out = (GrnModelOut *)B.out;
     l.fields = fields ? (double *)fields->p : nullptr;
     l.field_stride = field_stride;
     l.field_exp = field_exp;
@@ -557,7 +567,7 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
   // event; the same pattern is captured into a graph).
   const bool fork = P.launches.size() > 1;
   auto issue = [&]() -> int {
-    HIP_TRY(grn_launch_clear_results((GrnBatch *)B.batch.p, (GrnModelOut *)B.out.p, st));
+    HIP_TRY(grn_launch_clear_results((GrnBatch *)B.batch, (GrnModelOut *)B.out, st));
     if (fork) {
       HIP_TRY(hipEventRecord(c->ev_fork, st));
       HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
@@ -567,7 +577,7 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
       const Plan::Launch &x = P.launches[gi];
       const GrnProgram *pp = L[gi].progs;
       const GrnDataset *dp = L[gi].datasets;
-      const int *op = (const int *)B.order.p;
+      const int *op = (const int *)B.order;
       void *args[] = {&L[gi], &pp, &dp, &op};
       e = hipLaunchKernel(x.kc.k->fn, dim3(x.grid), dim3(x.kc.block), args, x.kc.lds, gi % 2 ? c->stream2 : st);
     }
@@ -580,9 +590,9 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
   };
   if (c->launch_mode == GRN_LAUNCH_GRAPH) {
     // the layout: everything a captured launch bakes in
-    std::vector<long> key = {(long)B.batch.p, (long)B.out.p, (long)B.progs.p,
-                             (long)(P.p64.empty() ? nullptr : B.progs64.p), (long)B.order.p,
-                             (long)B.maxerr.p, (long)c->d_datasets.p, (long)B.scratch.p,
+    std::vector<long> key = {(long)B.batch, (long)B.out, (long)B.progs,
+                             (long)(P.p64.empty() ? nullptr : B.progs64), (long)B.order,
+                             (long)B.maxerr, (long)c->d_datasets.p, (long)B.scratch,
                              (long)(fields ? fields->p : nullptr), field_stride, field_exp, P.step_cap};
     for (const Plan::Launch &x : P.launches) {
       key.insert(key.end(), {(long)x.kc.k->fn, x.grid, x.kc.block, (long)x.kc.lds, x.kc.slot_floats,
@@ -623,12 +633,17 @@ int launch_plan(Ctx *c, const Plan &P, BatchBufs &B, DevBuf *fields, long field_
 // Waits for the batch, turns the kernels' outputs into results
 // (CEvaluatorMorphogens::evaluate, cevaluatormorphogens.cpp:239-242) and
 // records the context's statistics.
-int collect_plan(Ctx *c, const Plan &P, BatchBufs &B, bool replayed, grn_result *out, int *n_products_out,
-                 std::chrono::steady_clock::time_point t_start) {
+int collect_plan(Ctx *c, const Plan &P, const DevPtrs &B, bool replayed, grn_result *out, int *n_products_out,
+                 std::chrono::steady_clock::time_point t_start, GrnModelOut *host_res = nullptr) {
   const int n = P.n;
   hipStream_t st = c->stream;
-  std::vector<GrnModelOut> res(n);
-  if (n > 0) HIP_TRY(hipMemcpyAsync(res.data(), B.out.p, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
+  std::vector<GrnModelOut> res_own;
+  if (!host_res) {
+    res_own.resize(n);
+    host_res = res_own.data();
+  }
+  const GrnModelOut *res = host_res;
+  if (n > 0) HIP_TRY(hipMemcpyAsync(host_res, B.out, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   float ms = 0.f;
   if (n > 0) HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -689,7 +704,8 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
   std::vector<double> maxerr;
   bool replayed = false;
   if ((rc = upload_plan(c, P, c->bufs, &maxerr)) ||
-      (rc = launch_plan(c, P, c->bufs, fields_host ? &c->d_fields : nullptr, field_stride, field_exp, &replayed))) {
+      (rc = launch_plan(c, P, ptrs_of(c->bufs), fields_host ? &c->d_fields : nullptr, field_stride, field_exp,
+                        &replayed))) {
     (void)hipStreamSynchronize(c->stream);  // the uploads read host buffers of this frame
     return rc;
   }
@@ -708,7 +724,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
                     sizeof(double) * field_stride);
       }
   }
-  return collect_plan(c, P, c->bufs, replayed, out, n_products_out, t_start);
+  return collect_plan(c, P, ptrs_of(c->bufs), replayed, out, n_products_out, t_start);
 }
 
 // A batch compiled, planned and uploaded once (grn_stage_batch), launched by
@@ -960,12 +976,12 @@ int grn_run_staged(void *ctx, void *staged, grn_result *out) {
   HIP_TRY(hipSetDevice(c->device));
   const auto t_start = std::chrono::steady_clock::now();
   bool replayed = false;
-  int rc = launch_plan(c, S->P, S->B, nullptr, 0, 0, &replayed);
+  int rc = launch_plan(c, S->P, ptrs_of(S->B), nullptr, 0, 0, &replayed);
   if (rc) {
     (void)hipStreamSynchronize(c->stream);
     return rc;
   }
-  return collect_plan(c, S->P, S->B, replayed, out, nullptr, t_start);
+  return collect_plan(c, S->P, ptrs_of(S->B), replayed, out, nullptr, t_start);
 }
 
 void grn_staged_destroy(void *staged) {

@@ -61,6 +61,33 @@ class Workload:
     target: np.ndarray | None = None  # [1][n_tgt][H][W], filled by make_target
 
 
+def ensure_activated(model: Model, target_gene: int, n_inputs: int, seed: int) -> Model:
+    """A hidden network whose target gene is activated by no input gene may
+    produce zero there everywhere (h = 0 without an activator, modelsimulator.cpp:
+    120-138, and a self-activation from zero stays at zero), which makes every
+    model's fitness against it trivial.  Such a network gets one OR
+    activation of the target gene from an input gene that does not regulate
+    it yet, drawn from its own stream (seed), so the workload's other draws
+    are unchanged."""
+    links = model.links
+    into = links[links["regulated"] == target_gene]
+    if np.any((into["hill_coef"] >= 0) & (into["regulator"] < n_inputs)):
+        return model
+    rng = np.random.default_rng([seed, 0x5EED])
+    free = [i for i in range(n_inputs) if i not in set(int(r) for r in into["regulator"])]
+    if not free:  # every input already represses it: turn the first such link around
+        l = links.copy()
+        k = int(np.where((l["regulated"] == target_gene) & (l["regulator"] < n_inputs))[0][0])
+        l["hill_coef"][k] = abs(l["hill_coef"][k])
+        return Model(model.genes.copy(), l)
+    new = np.zeros(1, LINK_DTYPE)
+    new[0] = (int(rng.choice(free)), target_gene, rng.uniform(0.05, 1.0), rng.uniform(1.0, 4.0), 0)
+    l = np.zeros(len(links) + 1, LINK_DTYPE)
+    l[:len(links)] = links
+    l[len(links):] = new
+    return Model(model.genes.copy(), l)
+
+
 def _population(rng, n_models, g_range, l_range, n_struct, **kw):
     out = []
     for _ in range(n_models):
@@ -78,6 +105,7 @@ def config0(seed: int = 0, n_models: int = 64) -> Workload:
     init = np.stack([gradient_x(W, H), gradient_y(W, H)])[None]
     hidden = random_model(rng, 3, int(rng.integers(2, 7)))
     models = _population(rng, n_models, (3, 3), (2, 6), 3)
+    hidden = ensure_activated(hidden, 2, 2, seed)
     return Workload("config0: 64 models, 3 genes, 32x32, 1000 steps", p, init, hidden, models)
 
 
@@ -95,6 +123,7 @@ def config1(seed: int = 0, n_models: int = 4096) -> Workload:
     init = np.stack([gradient_x(W, H), gradient_y(W, H), gaussian_blob(W, H, cx, cy, 8.0)])[None]
     hidden = random_model(rng, 6, int(rng.integers(6, 19)))
     models = _population(rng, n_models, (3, 8), (2, 24), 4)
+    hidden = ensure_activated(hidden, 3, 3, seed)
     return Workload("config1: 4096 models, 3-8 genes, 2-24 links, 64x64, 2000 steps", p, init,
                     hidden, models)
 
