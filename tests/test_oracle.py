@@ -410,3 +410,13 @@ def test_decode_rgb_channels_and_y_flip():
     np.testing.assert_array_equal(ora.decode_rgb(img, 1)[0, 0], out[0, 0])
     with pytest.raises(ValueError):
         ora.decode_rgb(img, 4)
+
+
+def test_ranking_check_orders_only_distinct_fitness():
+    # the parity tests' ranking comparison: exact and near ties carry no
+    # order, every pair the reference tells apart must keep its order
+    from parity_util import same_ranking
+    want = np.array([0.5, 0.2, 0.2, 0.9, 0.5 * (1 + 1e-7)])
+    assert same_ranking(want * (1 + 1e-12), want)
+    assert same_ranking(np.array([0.5, 0.2 + 1e-12, 0.2, 0.9, 0.5]), want)   # ties may flip
+    assert not same_ranking(np.array([0.5, 0.95, 0.2, 0.9, 0.5]), want)     # 0.2 < 0.9 reversed
