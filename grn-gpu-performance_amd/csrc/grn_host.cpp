@@ -30,6 +30,26 @@
 #include "grn_model.hpp"
 #include "grn_program.h"
 
+// The batch buffers of grn_evaluate_batch_noalloc: one fixed array per device
+// (each device holds its own copy, at its own address, until the process
+// ends).  Layout: the queue table, then programs, work order, maxError bounds
+// and results for at most GRN_NOALLOC_MAX_MODELS models, each region aligned
+// to 256 bytes.  Host code never uses the array's name as an address: the
+// address comes from hipGetSymbolAddress, once per device.
+namespace noalloc {
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr int kMax = GRN_NOALLOC_MAX_MODELS;
+constexpr size_t kBatch = 0;
+constexpr size_t kProgs = align256(sizeof(GrnBatch));
+constexpr size_t kOrder = kProgs + align256(sizeof(GrnProgram) * kMax);
+constexpr size_t kMaxErr = kOrder + align256(sizeof(int) * kMax);
+constexpr size_t kOut = kMaxErr + align256(sizeof(double) * kMax);
+constexpr size_t kBytes = kOut + align256(sizeof(GrnModelOut) * kMax);
+static_assert(kBytes == GRN_NOALLOC_SCRATCH_BYTES, "grn.h states the scratch array's size");
+}  // namespace noalloc
+
+static __device__ __align__(256) unsigned char scratch[noalloc::kBytes];
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -143,6 +163,12 @@ struct Ctx {
   // returning, so the next batch can refill it.
   GrnProgram *h_progs = nullptr;
   size_t h_progs_cap = 0;
+  // grn_evaluate_batch_noalloc: this device's scratch array and host-side
+  // staging sized for its capacity at creation
+  void *noalloc = nullptr;
+  std::vector<GrnProgram> na_progs;
+  std::vector<double> na_maxerr;
+  std::vector<GrnModelOut> na_res;
 
   void drop_graphs() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
@@ -309,6 +335,23 @@ int choose_kernel(const Ctx &c, int W, int H, int G, int n_tgt, bool guard, int 
 }
 
 Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+// This device's address of the scratch array, looked up on the first context
+// created on the device (with the device current) and kept per device.
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_addr;
+
+int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    it = g_scratch_addr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
 
 int to_model(const grn_model *gm, grn::Model *m) {
   if (!grn::model_from_abi(gm, m)) return fail(GRN_E_INVALID, "invalid grn_model");
@@ -535,7 +578,7 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
 // then one persistent kernel per class, eagerly or as a captured graph,
 // bracketed by the context's timing events.
 int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long field_stride, int field_exp,
-                bool *replayed) {
+                bool *replayed, bool allow_graph = true) {
   *replayed = false;
   if (P.n == 0) return 0;
   std::vector<GrnLaunch> L(P.launches.size());
@@ -588,7 +631,7 @@ int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long fi
     if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipLaunchKernel failed: %s", hipGetErrorString(e));
     return 0;
   };
-  if (c->launch_mode == GRN_LAUNCH_GRAPH) {
+  if (c->launch_mode == GRN_LAUNCH_GRAPH && allow_graph) {
     // the layout: everything a captured launch bakes in
     std::vector<long> key = {(long)B.batch, (long)B.out, (long)B.progs,
                              (long)(P.p64.empty() ? nullptr : B.progs64), (long)B.order,
@@ -806,6 +849,11 @@ int grn_create(int device, void **out) {
   // Static LDS of the kernels (queue slot, per-wave partials) stays below 1 KB.
   c->max_lds = lds > 1024 ? (size_t)lds - 1024 : 0;
   collect_kernels(&c->kernels);
+  int rc = scratch_address(device, &c->noalloc);
+  if (rc) return rc;
+  c->na_progs.resize(noalloc::kMax);
+  c->na_maxerr.resize(noalloc::kMax);
+  c->na_res.resize(noalloc::kMax);
   *out = c.release();
   return 0;
 }
@@ -942,6 +990,43 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids, const gr
   if (path < GRN_PATH_AUTO || path > GRN_PATH_EXACT) return fail(GRN_E_INVALID, "unknown path %d", path);
   HIP_TRY(hipSetDevice(c->device));
   return run_batch(c, n_models, dataset_ids, models, max_errors, dt_override, path, out, nullptr, 0, 0, nullptr);
+}
+
+int grn_evaluate_batch_noalloc(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
+                               const double *max_errors, const double *dt_override, grn_result *out) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  if (n_models < 0 || (n_models > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
+  if (n_models > noalloc::kMax)
+    return fail(GRN_E_CAPACITY, "grn_evaluate_batch_noalloc takes at most %d models, got %d", noalloc::kMax, n_models);
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t_start = std::chrono::steady_clock::now();
+  static thread_local Plan plan;  // host containers keep their capacity from call to call
+  int rc = make_plan(c, n_models, dataset_ids, models, max_errors, dt_override, GRN_PATH_LDS, BatchOptions(), &plan);
+  if (rc) return rc;
+  if (plan.scratch_bytes || !plan.p64.empty())
+    return fail(GRN_E_CAPACITY, "a model needs device memory beyond the fixed scratch array");
+  unsigned char *base = static_cast<unsigned char *>(c->noalloc);
+  const DevPtrs D{base + noalloc::kProgs, nullptr,       base + noalloc::kOrder, base + noalloc::kMaxErr,
+                  base + noalloc::kOut,   base + noalloc::kBatch, nullptr};
+  const int n = n_models;
+  for (int i = 0; i < n; ++i) {
+    c->na_progs[i] = plan.items[i].prog;
+    c->na_maxerr[i] = plan.items[i].max_err;
+  }
+  hipStream_t st = c->stream;
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(D.progs, c->na_progs.data(), sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(D.order, plan.idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(D.maxerr, c->na_maxerr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(D.batch, &plan.batch, sizeof(GrnBatch), hipMemcpyHostToDevice, st));
+  bool replayed = false;
+  if ((rc = launch_plan(c, plan, D, nullptr, 0, 0, &replayed, false))) {
+    (void)hipStreamSynchronize(st);  // the uploads read host arrays
+    return rc;
+  }
+  return collect_plan(c, plan, D, replayed, out, nullptr, t_start, c->na_res.data());
 }
 
 int grn_stage_batch(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,

@@ -29,6 +29,7 @@ PATH_AUTO, PATH_LDS, PATH_TILED, PATH_EXACT = 0, 1, 2, 3
 LAUNCH_EAGER, LAUNCH_GRAPH = 0, 1
 MAX_GENES = 16
 MAX_LINKS = 96
+NOALLOC_MAX_MODELS = 256
 
 GENE_DTYPE = np.dtype({"names": ["label", "init", "lim", "deg", "dif"],
                        "formats": [np.int32, np.float64, np.float64, np.float64, np.float64],
@@ -39,7 +40,8 @@ LINK_DTYPE = np.dtype({"names": ["regulator", "regulated", "dis_const", "hill_co
 
 EXPORTED = [
     "grn_abi_version", "grn_last_error", "grn_device_count", "grn_create", "grn_destroy",
-    "grn_add_dataset", "grn_decode_rgb", "grn_add_dataset_rgb", "grn_evaluate", "grn_evaluate_batch", "grn_stage_batch", "grn_run_staged",
+    "grn_add_dataset", "grn_decode_rgb", "grn_add_dataset_rgb", "grn_evaluate", "grn_evaluate_batch",
+    "grn_evaluate_batch_noalloc", "grn_stage_batch", "grn_run_staged",
     "grn_staged_destroy", "grn_simulate_fields",
     "grn_get_stats", "grn_test_performance", "grn_set_launch_mode", "grn_autotune", "grn_apply_tuning", "grn_clear_tuning",
     "grn_ceil_s", "grn_model_parse", "grn_model_format", "grn_model_info",
@@ -215,6 +217,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_evaluate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Model), ctypes.c_double,
                                  ctypes.POINTER(_Result)]
     lib.grn_evaluate_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp]
+    lib.grn_evaluate_batch_noalloc.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
     lib.grn_stage_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
     lib.grn_run_staged.argtypes = [vp, vp, vp]
     lib.grn_staged_destroy.argtypes = [vp]
@@ -510,6 +513,24 @@ class Context:
             self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
             None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data,
             path, out.ctypes.data))
+        return out[:n]
+
+    def evaluate_batch_noalloc(self, models, datasets=None, max_errors=None, dt=None) -> np.ndarray:
+        """grn_evaluate_batch_noalloc: the on-chip batched path with every batch
+        buffer in the device's fixed scratch array (at most NOALLOC_MAX_MODELS
+        models; no device allocation)."""
+        n = len(models)
+        cms = models.array if isinstance(models, ModelBatch) else (_Model * max(n, 1))(*[m._c() for m in models])
+        ds = None if datasets is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(datasets, np.int32), (n,)))
+        me = None if max_errors is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(max_errors, np.float64), (n,)))
+        dts = None if dt is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(dt, np.float64), (n,)))
+        out = np.zeros(max(n, 1), RESULT_DTYPE)
+        _check(self._lib.grn_evaluate_batch_noalloc(
+            self._h, n, None if ds is None else ds.ctypes.data, ctypes.addressof(cms),
+            None if me is None else me.ctypes.data, None if dts is None else dts.ctypes.data, out.ctypes.data))
         return out[:n]
 
     def stage_batch(self, models, datasets=None, max_errors=None, dt=None,

@@ -189,6 +189,25 @@ int grn_evaluate_batch(void *ctx, int n_models, const int *dataset_ids,
                        const grn_model *models, const double *max_errors,
                        const double *dt_override, int path, grn_result *out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  grn_evaluate_batch with GRN_PATH_LDS on at most
+ * GRN_NOALLOC_MAX_MODELS models, with every batch buffer (programs, work
+ * order, maxError bounds, results, work-queue table) placed in one fixed
+ * array per device of GRN_NOALLOC_SCRATCH_BYTES bytes, which each device
+ * holds until the process ends.  It calls no hipMalloc, hipFree or pinned
+ * allocation; its host staging is sized when the context is created.
+ * Results equal grn_evaluate_batch(..., GRN_PATH_LDS, ...) on the same
+ * arguments.  More models, or a model that needs the halo-tiled or the FP64
+ * kernel (device memory per workgroup), fail with GRN_E_CAPACITY.  Every
+ * call on a device shares that device's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (any contexts, any threads) race.
+ * Launches are always eager (no graph is captured). */
+#define GRN_NOALLOC_MAX_MODELS 256
+#define GRN_NOALLOC_SCRATCH_BYTES 545024
+int grn_evaluate_batch_noalloc(void *ctx, int n_models, const int *dataset_ids, const grn_model *models,
+                               const double *max_errors, const double *dt_override, grn_result *out);
+
 /* A batch compiled, planned and uploaded once: grn_run_staged evaluates it
  * again and again with the programs resident in device memory (only the
  * results cross the bus), giving the same results as grn_evaluate_batch on
