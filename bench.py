@@ -32,11 +32,23 @@ sys.path.insert(0, os.path.join(ROOT, "grn-gpu-performance_amd"))
 import numpy as np  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per batched launch of this workload: FETCH_SIZE x 2.0 + WRITE_SIZE
-# x 1.0 (scale factors from the copy-kernel calibration) as measured by
-# rocprofv3 --pmc; the commands and raw counts are in
-# profiles/field_kernel_config1_64x64.md.
-MEASURED_TRAFFIC_BYTES = 1545.3e6
+# HBM bytes per batched launch of the headline workload: FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (scale factors from the copy-kernel calibration) from the
+# two rocprofv3 --pmc runs recorded, with their commands and raw counts, in
+# this profile (its "traffic_bytes_per_launch:" line).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "field_kernel_config1_64x64.md")
+
+
+def profiled_traffic():
+    """The HBM bytes per launch the committed profile measured, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -92,24 +104,23 @@ def cpu_baseline(w, gpu_res=None, budget_s=22.0):
             "parity_vs_gpu": parity}
 
 
-def bench_islands(args):
-    """Multi-island evolution on one GPU: every generation (simulation and
-    fitness of all islands' children in one batch of persistent launches
-    under a captured HIP graph, then selection and ring migration) is one
-    step."""
+def measure_islands(ctx, seed, islands, island_size, warmup, steps, tune=True, tune_sample=256, tune_steps=100):
+    """Multi-island evolution generations (16 islands x 1024 models, sides
+    32-128, 3000 steps): each generation is the simulation and fitness of
+    every island's children in one graph-captured batch of persistent
+    launches, then selection and ring migration.  Returns a dict of rates."""
     import grn
     import workloads
 
-    w = workloads.config2(seed=args.seed, n_islands=args.islands, island_size=args.island_size)
-    ctx = grn.Context(0)
+    w = workloads.config2(seed=seed, n_islands=islands, island_size=island_size)
     ds = workloads.add_island_datasets(ctx, w)
     search = grn.Search(island_datasets=ds, **w.search)
     tuning = []
-    if not args.no_tune:
+    if tune:
         # tile and workgroup shapes per size class, timed on the initial population
         models, dsi, _, _ = search.pending()
         t0 = time.perf_counter()
-        reports = ctx.autotune(models, datasets=dsi, sample_models=args.tune_sample, step_cap=args.tune_steps)
+        reports = ctx.autotune(models, datasets=dsi, sample_models=tune_sample, step_cap=tune_steps)
         log(f"[bench] autotuned {len(reports)} size classes in {time.perf_counter() - t0:.1f} s")
         for r in reports:
             c, d = r["cand"][r["chosen"]], r["cand"][0]
@@ -121,13 +132,13 @@ def bench_islands(args):
                            "candidates": r["n_candidates"],
                            "speedup_vs_default": d["ms"] / c["ms"] if d["ms"] > 0 else None})
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
-    log(f"[bench] {w.name}: warmup x{args.warmup}")
-    for _ in range(args.warmup):
+    log(f"[bench] {w.name}: warmup x{warmup}")
+    for _ in range(warmup):
         search.run_generation(ctx)
-    log(f"[bench] timing x{args.steps} generations")
+    log(f"[bench] timing x{steps} generations")
     cell_steps, alg_flops, kernel_ms, replays, gens = 0.0, 0.0, [], 0, []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         g = search.run_generation(ctx)
         st = ctx.stats()
         cell_steps += st["cell_steps"]
@@ -136,32 +147,129 @@ def bench_islands(args):
         replays += st["graph_replayed"]
         gens.append(g)
     wall = time.perf_counter() - t0
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
     k = float(np.sum(kernel_ms)) / 1e3
     achieved = alg_flops / k / 1e12
+    out = {"workload": "multi-island evolution, config 2: " + w.name, "sides": w.sides,
+           "cell_steps_per_s": cell_steps / wall, "kernel_cell_steps_per_s": cell_steps / k,
+           "ms_per_generation": wall / steps * 1e3, "kernel_ms_per_generation": k / steps * 1e3,
+           "alg_tflops": achieved, "frac_of_fp32_peak": achieved / FP32_PEAK_TFLOPS,
+           "launches_per_generation": ctx.stats()["n_launches"], "graph_replays": replays,
+           "best_fitness": gens[-1]["min_fit"], "autotune": tuning}
+    search.close()
+    return out
+
+
+def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps=1):
+    """Generations of the full-evolution workload at its real grid (100x100,
+    5000 steps, four inputs, hidden 10-gene target): n_islands x island_size
+    models per generation, plain or screened by the learned surrogate (4
+    candidates per slot, the better-predicted half kept).  Returns a dict."""
+    import grn
+    import evolve
+
+    prob = evolve.scaled_config4(seed=seed, n_islands=n_islands, island_size=island_size, side=100, steps=5000)
+    evolve.make_target(ctx, prob)
+    ds = ctx.add_dataset(prob.params, prob.init, prob.target)
+    search = grn.Search(island_datasets=[ds] * n_islands, **prob.search)
+    sur = None
+    if screened:
+        sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], seed=prob.search["seed"])
+        search.set_oversample(4)
+        search.set_screen_policy(grn.SCREEN_HALF)
+    ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+    # generation 0 (the initial population) is simulated unscreened
+    t0 = time.perf_counter()
+    g0 = search.run_generation_screened(ctx, sur, train_steps=10)[0] if screened else search.run_generation(ctx)
+    gen0_s = time.perf_counter() - t0
+    st0 = ctx.stats()
+    for _ in range(warmup):
+        search.run_generation_screened(ctx, sur, train_steps=10) if screened else search.run_generation(ctx)
+    cell_steps, kernel_ms = 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if screened:
+            g, sc = search.run_generation_screened(ctx, sur, train_steps=10)
+        else:
+            g = search.run_generation(ctx)
+        st = ctx.stats()
+        cell_steps += st["cell_steps"]
+        kernel_ms += st["kernel_ms"]
+    wall = time.perf_counter() - t0
+    ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    out = {"models_per_generation": n_islands * island_size, "screened": screened,
+           "generation0_s": gen0_s, "generation0_cell_steps_per_s": st0["cell_steps"] / gen0_s,
+           "ms_per_generation": wall / steps * 1e3, "cell_steps_per_s": cell_steps / wall,
+           "kernel_cell_steps_per_s": cell_steps / (kernel_ms / 1e3), "best_fitness": g["min_fit"]}
+    if sur is not None:
+        out["spearman"] = sc["spearman"]
+        sur.close()
+    search.close()
+    return out
+
+
+def bench_islands(args):
+    """The multi-island generation as the bench line's workload (one
+    generation is one step)."""
+    import grn
+
+    ctx = grn.Context(0)
+    r = measure_islands(ctx, args.seed, args.islands, args.island_size, args.warmup, args.steps,
+                        tune=not args.no_tune, tune_sample=args.tune_sample, tune_steps=args.tune_steps)
     out = {
         "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
-        "value": cell_steps / wall,
+        "value": r["cell_steps_per_s"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms_per_generation"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "multi-island evolution, config 2: " + w.name, "islands": args.islands,
-                   "island_size": args.island_size, "sides": w.sides, "sim_steps": w.params[0].num_sim_steps,
+        "config": {"workload": r["workload"], "islands": args.islands,
+                   "island_size": args.island_size, "sides": r["sides"], "sim_steps": 3000,
                    "parallelism": "one graph-captured batch of persistent launches per generation"},
-        "kernel_cell_steps_per_s": cell_steps / k,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": float(np.mean(kernel_ms))},
-        "graph_replays": replays,
-        "autotune": tuning,
-        "launches_per_generation": ctx.stats()["n_launches"],
-        "best_fitness": gens[-1]["min_fit"],
+        "kernel_cell_steps_per_s": r["kernel_cell_steps_per_s"],
+        "roofline": {"bound": "valu", "achieved": r["alg_tflops"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": r["frac_of_fp32_peak"], "traffic": None,
+                     "kernel_ms": r["kernel_ms_per_generation"]},
+        "graph_replays": r["graph_replays"],
+        "autotune": r["autotune"],
+        "launches_per_generation": r["launches_per_generation"],
+        "best_fitness": r["best_fitness"],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+def bench_config4(args):
+    """Generations of the full-evolution workload at its real size (32
+    islands x 2048 models, 100x100, 5000 steps), plain and surrogate-screened
+    on the same seed: seconds per generation and cell-steps/s.  Time to
+    fitness at this size (hundreds of generations of about a minute) is not
+    measured here; --workload evolve measures it at a reduced size."""
+    import grn
+
+    ctx = grn.Context(0)
+    plain = measure_config4(ctx, args.seed, args.islands4, args.island_size4, False, steps=args.steps)
+    log(f"[bench] plain: {plain}")
+    scr = measure_config4(ctx, args.seed, args.islands4, args.island_size4, True, steps=args.steps)
+    log(f"[bench] screened: {scr}")
+    out = {
+        "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
+        "value": plain["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": plain["ms_per_generation"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"full evolution, config 4: {args.islands4} islands x {args.island_size4}, "
+                               "100x100, 5000 steps, hidden 10-gene target",
+                   "parallelism": "one graph-captured batch of persistent launches per generation"},
+        "plain": plain, "screened": scr,
+        "screened_over_plain_generation_time": plain["ms_per_generation"] / scr["ms_per_generation"],
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -314,7 +422,11 @@ def main():
     ap.add_argument("--coarse-side", type=int, default=8, help="evolve --screen coarse: side of the coarse grid")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["config1", "config2", "evolve", "surrogate"], default="config1",
+    ap.add_argument("--no-extra", action="store_true",
+                    help="config1: skip the config-2 generation and config-4 island reported beside the headline")
+    ap.add_argument("--islands4", type=int, default=32, help="config4: islands")
+    ap.add_argument("--island-size4", type=int, default=2048, help="config4: models per island")
+    ap.add_argument("--workload", choices=["config1", "config2", "config4", "evolve", "surrogate"], default="config1",
                     help="config1: one batch of 4096 models on 64x64 (the headline line); config2: "
                          "multi-island evolution generations on mixed grid sides; evolve: time to "
                          "fitness, plain vs surrogate-screened; surrogate: the MLP's training and "
@@ -341,6 +453,8 @@ def main():
 
     if args.workload == "config2":
         return bench_islands(args)
+    if args.workload == "config4":
+        return bench_config4(args)
     if args.workload == "evolve":
         return bench_time_to_fitness(args)
     if args.workload == "surrogate":
@@ -416,6 +530,15 @@ def main():
         log("[bench] CPU baseline")
         cpu = cpu_baseline(w, gpu_res=res)
 
+    # the other single-GPU workloads at their real sizes (reported beside the
+    # headline, not in `value`)
+    extra = {}
+    if not args.no_extra:
+        log("[bench] multi-island generation (16 islands x 1024, sides 32-128, 3000 steps)")
+        extra["config2_generation"] = measure_islands(ctx, args.seed, 16, 1024, 1, 1)
+        log("[bench] one island of the full-evolution workload (2048 models, 100x100, 5000 steps)")
+        extra["config4_island"] = measure_config4(ctx, args.seed, 1, 2048, False, steps=1)
+
     out = {
         "metric": "GRN cell-steps/s plain vs batched vs surrogate-screened; time-to-fitness",
         "value": value,
@@ -437,7 +560,7 @@ def main():
         "host_inclusive_cell_steps_per_s": host_inclusive,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC_BYTES if args.models == 4096 else None,
+                     "traffic": profiled_traffic() if args.models == 4096 else None,
                      "kernel_ms": float(np.mean(kernel_ms)),
                      "flops_per_cell_step_mean": float(np.mean(flops_per_model))},
         "autotune": tuning,
@@ -447,6 +570,7 @@ def main():
         "parity_max_rel": cpu["parity_vs_gpu"]["max_rel_fitness"] if cpu else None,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
     ctx.close()
 
