@@ -116,10 +116,12 @@ def coarse_dataset(ctx, prob: Problem, side: int):
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
            screen_policy: int = grn.SCREEN_ABSOLUTE, screen: str = "mlp", coarse_side: int = 8,
-           progress=None) -> Run:
+           progress=None, results=None, name: str = "") -> Run:
     """Plain (screened False) or screened evolution.  screen "mlp": the
     learned surrogate scores the candidates; "coarse": a simulation of every
-    candidate on a coarse_side x coarse_side grid does."""
+    candidate on a coarse_side x coarse_side grid does.  results (optional):
+    a store.Store that receives the run's parameters, every generation's
+    statistics and the final Pareto front."""
     if screened and screen not in ("mlp", "coarse"):
         raise ValueError(f"unknown screen {screen!r}")
     if prob.target is None:
@@ -139,6 +141,9 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         search.set_screen_fraction(screen_fraction)
         search.set_screen_policy(screen_policy)
     run = Run(screened)
+    if results is not None:
+        sid = results.begin_search(name or prob.name, prob.search["seed"], results.add_sim_params(prob.params),
+                                   results.add_search_params(search.params), n_isl)
     t0 = time.perf_counter()
     for gen in range(max_generations):
         if screened and screen == "coarse":
@@ -152,6 +157,8 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         run.cell_steps += ctx.stats()["cell_steps"]
         run.best.append(g["min_fit"])
         run.wall.append(time.perf_counter() - t0)
+        if results is not None:
+            results.add_generation(sid, g, run.wall[-1])
         if progress and gen % 100 == 0:
             progress(f"[evolve] {'screened' if screened else 'plain'} gen {gen} best {g['min_fit']:.4g} "
                      f"t {run.wall[-1]:.1f}s")
@@ -165,6 +172,9 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
     exact = float(ctx.evaluate_batch([best], datasets=ds, dt=[dt], path=grn.PATH_EXACT)["fitness"][0])
     run.verified = bool(abs(exact - err) <= 1e-5 * abs(exact))
     run.best_model, run.best_error, run.best_exact, run.best_dt = best, err, exact, dt
+    if results is not None:
+        results.add_pareto_front(sid, search.pareto())
+        results.end_search(sid)
     if sur is not None:
         sur.close()
     search.close()

@@ -28,8 +28,17 @@ def test_plain_and_screened_evolution_are_verified_and_deterministic():
     ctx = grn.Context(0)
     prob = evolve.scaled_config4(seed=2, n_islands=4, island_size=128, steps=400)
     evolve.make_target(ctx, prob)
-    a = evolve.evolve(ctx, prob, False, max_generations=12, threshold=0.0)
+    import os
+    import tempfile
+    import store
+    path = os.path.join(tempfile.mkdtemp(), "evolve.db")
+    with store.Store(path) as results:
+        a = evolve.evolve(ctx, prob, False, max_generations=12, threshold=0.0, results=results, name="plain")
     b = evolve.evolve(ctx, prob, False, max_generations=12, threshold=0.0)
+    with store.Store(path) as results:  # the run's record reads back
+        gens = results.generations(1)
+        assert [g["min_fit"] for g in gens] == a.best
+        assert results.pareto_front(1)[0][1] == a.best[-1]
     assert a.best == b.best and a.verified and b.verified
     assert all(x >= y for x, y in zip(a.best, a.best[1:]))
     s = evolve.evolve(ctx, prob, True, max_generations=12, threshold=0.0, oversample=4, train_steps=5)
