@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "grn.h"
+#include "grn_internal.hpp"
 #include "grn_kernels.h"
 #include "grn_model.hpp"
 #include "grn_program.h"
@@ -163,6 +164,24 @@ struct Ctx {
   // returning, so the next batch can refill it.
   GrnProgram *h_progs = nullptr;
   size_t h_progs_cap = 0;
+  // A batch with a host stage (evaluate_batch_hooked): its results are copied
+  // into pinned memory and a host function -- result conversion plus the
+  // caller's hook -- runs in stream order after the launches, inside the
+  // captured graph in graph mode.  The host function reads `hook`, whose
+  // address the graph bakes in and whose fields are set before each launch.
+  struct Hook {
+    Ctx *c = nullptr;
+    const void *plan = nullptr;
+    grn_result *out = nullptr;
+    int *n_products = nullptr;
+    grn::BatchHook fn = nullptr;
+    void *user = nullptr;
+    int rc = 0;
+    std::string err;
+    double cell_steps = 0, flops = 0;
+  } hook;
+  GrnModelOut *h_out = nullptr;
+  size_t h_out_cap = 0;
   // grn_evaluate_batch_noalloc: this device's scratch array and host-side
   // staging sized for its capacity at creation
   void *noalloc = nullptr;
@@ -188,6 +207,7 @@ struct Ctx {
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
     if (h_progs) (void)hipHostFree(h_progs);
+    if (h_out) (void)hipHostFree(h_out);
   }
 };
 
@@ -574,11 +594,13 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
   return 0;
 }
 
+void hook_host_fn(void *p);
+
 // Enqueues the batch: result clearing (which also rewinds the work queues),
 // then one persistent kernel per class, eagerly or as a captured graph,
 // bracketed by the context's timing events.
 int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long field_stride, int field_exp,
-                bool *replayed, bool allow_graph = true) {
+                bool *replayed, bool allow_graph = true, bool hooked = false) {
   *replayed = false;
   if (P.n == 0) return 0;
   std::vector<GrnLaunch> L(P.launches.size());
@@ -629,6 +651,10 @@ int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long fi
       HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
     }
     if (e != hipSuccess) return fail(GRN_E_DEVICE, "hipLaunchKernel failed: %s", hipGetErrorString(e));
+    if (hooked) {  // the host stage, in stream order: results to pinned memory, then the host function
+      HIP_TRY(hipMemcpyAsync(c->h_out, B.out, sizeof(GrnModelOut) * P.n, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipLaunchHostFunc(st, hook_host_fn, &c->hook));
+    }
     return 0;
   };
   if (c->launch_mode == GRN_LAUNCH_GRAPH && allow_graph) {
@@ -636,7 +662,8 @@ int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long fi
     std::vector<long> key = {(long)B.batch, (long)B.out, (long)B.progs,
                              (long)(P.p64.empty() ? nullptr : B.progs64), (long)B.order,
                              (long)B.maxerr, (long)c->d_datasets.p, (long)B.scratch,
-                             (long)(fields ? fields->p : nullptr), field_stride, field_exp, P.step_cap};
+                             (long)(fields ? fields->p : nullptr), field_stride, field_exp, P.step_cap,
+                             hooked ? (long)c->h_out : 0L, hooked ? (long)P.n : -1L};
     for (const Plan::Launch &x : P.launches) {
       key.insert(key.end(), {(long)x.kc.k->fn, x.grid, x.kc.block, (long)x.kc.lds, x.kc.slot_floats,
                              x.kc.tile.tw, x.kc.tile.th, x.kc.tile.sx, x.kc.tile.sy, x.kc.tile.halo});
@@ -676,26 +703,22 @@ int launch_plan(Ctx *c, const Plan &P, const DevPtrs &B, DevBuf *fields, long fi
 // Waits for the batch, turns the kernels' outputs into results
 // (CEvaluatorMorphogens::evaluate, cevaluatormorphogens.cpp:239-242) and
 // records the context's statistics.
-int collect_plan(Ctx *c, const Plan &P, const DevPtrs &B, bool replayed, grn_result *out, int *n_products_out,
-                 std::chrono::steady_clock::time_point t_start, GrnModelOut *host_res = nullptr) {
+// Turns the kernels' outputs into results (CEvaluatorMorphogens::evaluate,
+// cevaluatormorphogens.cpp:239-242): the maxError sentinel and ceilS.  Pure
+// host code with no HIP call, so it also runs as a graph's host function;
+// it reports failures through its return value and *err.
+int convert_results(const Ctx *c, const Plan &P, const GrnModelOut *res, grn_result *out, int *n_products_out,
+                    double *cell_steps_out, double *flops_out, std::string *err) {
   const int n = P.n;
-  hipStream_t st = c->stream;
-  std::vector<GrnModelOut> res_own;
-  if (!host_res) {
-    res_own.resize(n);
-    host_res = res_own.data();
-  }
-  const GrnModelOut *res = host_res;
-  if (n > 0) HIP_TRY(hipMemcpyAsync(host_res, B.out, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  float ms = 0.f;
-  if (n > 0) HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   double cell_steps = 0, flops = 0;
   for (int i = 0; i < n; ++i) {
     const Item &it = P.items[i];
     const HostDataset &D = c->datasets[it.dataset];
     const GrnModelOut &o = res[i];
-    if (!o.done) return fail(GRN_E_RESULT, "model %d: the kernel wrote no result", i);
+    if (!o.done) {
+      *err = "model " + std::to_string(i) + ": the kernel wrote no result";
+      return GRN_E_RESULT;
+    }
     grn_result &r = out[i];
     r.raw_error = o.err;
     r.change = o.change;
@@ -714,6 +737,48 @@ int collect_plan(Ctx *c, const Plan &P, const DevPtrs &B, bool replayed, grn_res
     flops += cs * it.info.flops_per_cell_step;
     if (n_products_out) n_products_out[i] = it.prog.G;
   }
+  *cell_steps_out = cell_steps;
+  *flops_out = flops;
+  return 0;
+}
+
+// The host stage of a hooked batch (see Ctx::hook).
+void hook_host_fn(void *p) {
+  Ctx::Hook *h = static_cast<Ctx::Hook *>(p);
+  const Plan &P = *static_cast<const Plan *>(h->plan);
+  h->err.clear();
+  h->rc = convert_results(h->c, P, h->c->h_out, h->out, h->n_products, &h->cell_steps, &h->flops, &h->err);
+  if (!h->rc && h->fn) h->rc = h->fn(h->user, h->out, P.n, &h->err);
+}
+
+// Waits for the batch and records the context's statistics.  Without a
+// host stage the results are copied back and converted here; with one
+// (hooked) the launches already copied and converted them.
+int collect_plan(Ctx *c, const Plan &P, const DevPtrs &B, bool replayed, grn_result *out, int *n_products_out,
+                 std::chrono::steady_clock::time_point t_start, GrnModelOut *host_res = nullptr,
+                 bool hooked = false) {
+  const int n = P.n;
+  hipStream_t st = c->stream;
+  std::vector<GrnModelOut> res_own;
+  if (!hooked && !host_res) {
+    res_own.resize(n);
+    host_res = res_own.data();
+  }
+  if (!hooked && n > 0)
+    HIP_TRY(hipMemcpyAsync(host_res, B.out, sizeof(GrnModelOut) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0.f;
+  if (n > 0) HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  double cell_steps = 0, flops = 0;
+  if (hooked) {
+    if (c->hook.rc) return fail(c->hook.rc, "%s", c->hook.err.c_str());
+    cell_steps = c->hook.cell_steps;
+    flops = c->hook.flops;
+  } else {
+    std::string err;
+    const int rc = convert_results(c, P, host_res, out, n_products_out, &cell_steps, &flops, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
+  }
   c->stats.kernel_ms = ms;
   c->stats.cell_steps = cell_steps;
   c->stats.alg_flops = flops;
@@ -731,7 +796,7 @@ int collect_plan(Ctx *c, const Plan &P, const DevPtrs &B, bool replayed, grn_res
 int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
               const double *max_errors, const double *dt_override, int path, grn_result *out,
               double *fields_host, long field_stride, int field_exp, int *n_products_out,
-              const BatchOptions &opt = BatchOptions()) {
+              const BatchOptions &opt = BatchOptions(), grn::BatchHook hook = nullptr, void *hook_user = nullptr) {
   const auto t_start = std::chrono::steady_clock::now();
   if (n < 0 || (n > 0 && (!models || !out))) return fail(GRN_E_INVALID, "invalid batch arguments");
   // one plan per host thread, reused: a batch's items hold 2 KB programs, and
@@ -744,11 +809,32 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
     if ((rc = c->d_fields.reserve(sizeof(double) * (size_t)field_stride * std::max(n, 1)))) return rc;
     if (P.groups.size() > 1) return fail(GRN_E_INVALID, "field dumps take one kernel class per call");
   }
+  const bool hooked = hook != nullptr && n > 0;
+  if (hooked) {
+    if (fields_host) return fail(GRN_E_INVALID, "a hooked batch does not dump fields");
+    if ((size_t)n > c->h_out_cap) {
+      HIP_TRY(hipStreamSynchronize(c->stream));  // no copy still writes the old buffer
+      if (c->h_out) (void)hipHostFree(c->h_out);
+      c->h_out = nullptr;
+      c->h_out_cap = 0;
+      c->drop_graphs();  // captured copies target the old buffer
+      const size_t cap = std::max<size_t>((size_t)n + n / 4, 1024);
+      HIP_TRY(hipHostMalloc((void **)&c->h_out, sizeof(GrnModelOut) * cap, hipHostMallocDefault));
+      c->h_out_cap = cap;
+    }
+    c->hook.c = c;
+    c->hook.plan = &P;
+    c->hook.out = out;
+    c->hook.n_products = n_products_out;
+    c->hook.fn = hook;
+    c->hook.user = hook_user;
+    c->hook.rc = 0;
+  }
   std::vector<double> maxerr;
   bool replayed = false;
   if ((rc = upload_plan(c, P, c->bufs, &maxerr)) ||
       (rc = launch_plan(c, P, ptrs_of(c->bufs), fields_host ? &c->d_fields : nullptr, field_stride, field_exp,
-                        &replayed))) {
+                        &replayed, true, hooked))) {
     (void)hipStreamSynchronize(c->stream);  // the uploads read host buffers of this frame
     return rc;
   }
@@ -767,7 +853,7 @@ int run_batch(Ctx *c, int n, const int *dataset_ids, const grn_model *models,
                     sizeof(double) * field_stride);
       }
   }
-  return collect_plan(c, P, ptrs_of(c->bufs), replayed, out, n_products_out, t_start);
+  return collect_plan(c, P, ptrs_of(c->bufs), replayed, out, n_products_out, t_start, nullptr, hooked);
 }
 
 // A batch compiled, planned and uploaded once (grn_stage_batch), launched by
@@ -794,6 +880,17 @@ bool staged_live(Staged *S) {
 }  // namespace
 
 namespace grn {
+
+int evaluate_batch_hooked(void *ctx, int n, const int *dataset_ids, const grn_model *models,
+                          const double *max_errors, const double *dt_override, int path, grn_result *out,
+                          BatchHook hook, void *user) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(GRN_E_INVALID, "null context");
+  if (path < GRN_PATH_AUTO || path > GRN_PATH_EXACT) return fail(GRN_E_INVALID, "unknown path %d", path);
+  HIP_TRY(hipSetDevice(c->device));
+  return run_batch(c, n, dataset_ids, models, max_errors, dt_override, path, out, nullptr, 0, 0, nullptr,
+                   BatchOptions(), hook, user);
+}
 
 int set_last_error(int code, const char *fmt, ...) {
   char buf[1024];

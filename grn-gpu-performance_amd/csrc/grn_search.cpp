@@ -16,6 +16,7 @@
 #include <string>
 #include <thread>
 
+#include "grn_internal.hpp"
 #include "grn_surrogate.hpp"
 
 namespace grn {
@@ -966,19 +967,41 @@ int grn_search_advance(void *search, const double *errors, grn_generation_stats 
   return 0;
 }
 
+// The host stage of a generation: selection (deterministic crowding), the
+// Pareto front, the statistics, migration when due and breeding of the next
+// children, from the simulated errors.  It runs in the batch's stream order
+// -- a host node of the captured graph when the context launches graphs --
+// so one generation is one graph launch.
+static int generation_stage(void *user, const grn_result *res, int n, std::string *err) {
+  SearchHandle *h = static_cast<SearchHandle *>(user);
+  h->errors.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const double e = res[i].fitness;
+    if (!(e >= 0) || std::isinf(e)) {
+      *err = "error " + std::to_string(i) + " is not finite and >= 0";
+      return GRN_E_INVALID;
+    }
+    h->errors[i] = e;
+  }
+  h->s->advance(h->errors.data());
+  h->flatten();
+  return 0;
+}
+
 int grn_search_run_generation(void *search, void *ctx, int path, grn_generation_stats *stats) {
   SearchHandle *h = as_search(search);
   if (!h || !ctx) return grn::set_last_error(GRN_E_INVALID, "null argument");
   if (h->pend.models.empty() && !h->s->candidates().empty())
     return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
   const int n = (int)h->pend.models.size();
+  if (n == 0) return grn_search_advance(search, nullptr, stats);
   h->results.resize(n);
-  h->errors.resize(n);
-  int rc = grn_evaluate_batch(ctx, n, h->pend.datasets.data(), h->pend.models.data(), h->pend.max_errors.data(),
-                              h->pend.dts.data(), path, h->results.data());
+  const int rc = grn::evaluate_batch_hooked(ctx, n, h->pend.datasets.data(), h->pend.models.data(),
+                                            h->pend.max_errors.data(), h->pend.dts.data(), path, h->results.data(),
+                                            generation_stage, h);
   if (rc) return rc;
-  for (int i = 0; i < n; ++i) h->errors[i] = h->results[i].fitness;
-  return grn_search_advance(search, h->errors.data(), stats);
+  if (stats) *stats = h->s->stats();
+  return 0;
 }
 
 int grn_search_set_oversample(void *search, int k) {

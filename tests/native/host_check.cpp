@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "grn.h"
+#include "grn_internal.hpp"
 #include "grn_model.hpp"
 #include "grn_program.h"
 #include "grn_search.hpp"
@@ -80,6 +81,15 @@ extern "C" int grn_evaluate_batch(void *, int n, const int *ds, const grn_model 
   }
   return 0;
 }
+
+namespace grn {
+int evaluate_batch_hooked(void *ctx, int n, const int *ds, const grn_model *models, const double *max_errors,
+                          const double *dt, int path, grn_result *out, BatchHook hook, void *user) {
+  grn_evaluate_batch(ctx, n, ds, models, max_errors, dt, path, out);
+  std::string err;
+  return hook ? hook(user, out, n, &err) : 0;
+}
+}  // namespace grn
 
 // ---- checks ------------------------------------------------------------------
 

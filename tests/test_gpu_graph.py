@@ -98,6 +98,37 @@ def test_search_through_captured_generations_equals_eager(ctx):
     print(f"[gpu] graph replays: {replays}", flush=True)
 
 
+def test_generation_with_host_stage_in_the_graph_equals_eager(ctx):
+    """grn_search_run_generation: selection, the Pareto front, statistics,
+    ring migration and breeding run as the host stage of the generation's
+    batch -- a host node of the captured graph in graph mode -- so a
+    generation is one graph launch.  Driven that way, a search follows the
+    same search run eagerly, and the search driven through the phased calls
+    (pending, evaluate_batch, advance) exactly."""
+    w = workloads.config2(seed=4, n_islands=3, island_size=32, steps=150, sides=[32, 48, 96])
+    ds = workloads.add_island_datasets(ctx, w)
+    params = dict(w.search, migration_period=2)
+    a = grn.Search(island_datasets=ds, **params)
+    b = grn.Search(island_datasets=ds, **params)
+    c = grn.Search(island_datasets=ds, **params)
+    replays = 0
+    for gen in range(7):
+        print(f"[gpu] generation {gen} with its host stage", flush=True)
+        ctx.set_launch_mode(grn.LAUNCH_GRAPH)
+        sa = a.run_generation(ctx)
+        replays += ctx.stats()["graph_replayed"]
+        ctx.set_launch_mode(grn.LAUNCH_EAGER)
+        sb = b.run_generation(ctx)
+        mc, dc, ec, tc = c.pending()
+        rc = ctx.evaluate_batch(mc, datasets=dc, max_errors=ec, dt=tc)
+        sc = c.advance(rc["fitness"])
+        assert sa == sb == sc, gen
+        pa, pb, pc = a.pending()[0], b.pending()[0], c.pending()[0]
+        assert [m.to_text(17) for m in pa] == [m.to_text(17) for m in pb] == [m.to_text(17) for m in pc]
+    assert replays >= 5  # every generation after the first replays the captured one
+    assert [x["error"] for x in a.pareto()] == [x["error"] for x in c.pareto()]
+
+
 def test_context_close_frees_staged_batches():
     """Closing a context with a staged batch still alive frees the batch
     first; a handle freed with its context is ignored by
