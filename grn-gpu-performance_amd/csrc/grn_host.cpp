@@ -546,7 +546,12 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc.k->fn, kc.block, kc.lds));
     if (per_cu < 1) return fail(GRN_E_CAPACITY, "kernel cannot be resident (block %d, LDS %zu)", kc.block, kc.lds);
-    const int grid = std::min(e - b, per_cu * c->n_cu);
+    // Eager launches start no more workgroups than the class has models; in
+    // graph mode every launch starts the full resident grid (the extra
+    // workgroups find the queue empty and exit), so a launch's layout does
+    // not change with its class's size and the captured graph is replayed
+    // for the next batch -- the next generation of a search.
+    const int grid = c->launch_mode == GRN_LAUNCH_GRAPH ? per_cu * c->n_cu : std::min(e - b, per_cu * c->n_cu);
     const size_t off = P->scratch_bytes / sizeof(float);
     P->scratch_bytes += sizeof(float) * (size_t)kc.slot_floats * grid;
     P->launches.push_back({kc, grid, off});
