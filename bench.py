@@ -179,6 +179,7 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
         search.set_screen_policy(grn.SCREEN_HALF)
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     # generation 0 (the initial population) is simulated unscreened
+    log(f"[bench] config 4: {n_islands} x {island_size} models, {'screened' if screened else 'plain'}, generation 0")
     t0 = time.perf_counter()
     g0 = search.run_generation_screened(ctx, sur, train_steps=10)[0] if screened else search.run_generation(ctx)
     gen0_s = time.perf_counter() - t0
@@ -187,7 +188,8 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
         search.run_generation_screened(ctx, sur, train_steps=10) if screened else search.run_generation(ctx)
     cell_steps, kernel_ms = 0.0, 0.0
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
+        log(f"[bench] config 4: timed generation {k + 1}")
         if screened:
             g, sc = search.run_generation_screened(ctx, sur, train_steps=10)
         else:
