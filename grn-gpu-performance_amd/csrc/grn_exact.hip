@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kNT) void exact_kernel(GrnLaunch a, const GrnProgra
 
 // wc = -2 marks the FP64 kernel; it applies the diffusion guard always.
 static const GrnKernelEntry kTable[] = {
-    {-2, 1, grn_exact::kNT, 1, 16, true, (const void *)&grn_exact::exact_kernel<1, 16>},
+    {-2, 1, grn_exact::kNT, 1, 16, true, (const void *)&grn_exact::exact_kernel<1, 16>, 0},
 };
 
 extern "C" const GrnKernelEntry *grn_field_table_exact(int *n) {

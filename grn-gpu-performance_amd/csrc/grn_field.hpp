@@ -549,4 +549,4 @@ __global__ __launch_bounds__(NTMAX) void field_kernel(GrnLaunch a, const GrnProg
 }  // namespace grn_field
 
 #define GRN_FIELD_ENTRY(WC, CPT, NTMAX, GLO, GHI, GUARD) \
-  {WC, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::field_kernel<WC, CPT, NTMAX, GLO, GHI, GUARD>}
+  {WC, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::field_kernel<WC, CPT, NTMAX, GLO, GHI, GUARD>, 0}

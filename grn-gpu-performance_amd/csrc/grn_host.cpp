@@ -225,10 +225,11 @@ void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
 }
 
 // Window of the halo-tiled kernel for a W x H grid: extended tiles of
-// tw x th cells (th a multiple of cpt, at most ntmax threads, G LDS planes
-// within the LDS budget), interior (tw - 2 halo) x (th - 2 halo).  Minimises
-// the cells computed per step over all tiles plus a per-tile overhead.
-bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int halo, TileGeom *out) {
+// tw x th cells (th a multiple of cpt, at most ntmax threads, tw at most the
+// kernel's fixed LDS pitch when it has one, G LDS planes within the LDS
+// budget), interior (tw - 2 halo) x (th - 2 halo).  Minimises the cells
+// computed per step over all tiles plus a per-tile overhead.
+bool tile_geometry(int W, int H, int G, int cpt, int ntmax, int pitch, size_t max_lds, int halo, TileGeom *out) {
   double best = 1e300;
   bool found = false;
   const int th_max = ((H + 2 * halo + cpt - 1) / cpt) * cpt;
@@ -238,8 +239,8 @@ bool tile_geometry(int W, int H, int G, int cpt, int ntmax, size_t max_lds, int 
     for (int tw = 2 * halo + 1; tw <= W + 2 * halo; ++tw) {
       const int sx = std::min(tw - 2 * halo, W);
       const long threads = (long)tw * (th / cpt);
-      if (threads > ntmax) break;
-      const size_t lds = (size_t)G * (tw + 2) * (th + 2) * sizeof(float);
+      if (threads > ntmax || (pitch > 0 && tw > pitch)) break;
+      const size_t lds = (size_t)G * ((pitch > 0 ? pitch : tw) + 2) * (th + 2) * sizeof(float);
       if (lds > max_lds) break;
       const int ntx = (W + sx - 1) / sx, nty = (H + sy - 1) / sy;
       const double cost = (double)ntx * nty * ((double)tw * th + 2048.0);
@@ -312,12 +313,12 @@ std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, i
       for (const GrnKernelEntry *k : c.kernels) {
         if (k->wc != -1 || G < k->glo || G > k->ghi || k->guard != guard) continue;
         TileGeom t;
-        if (!tile_geometry(W, H, G, k->cpt, k->ntmax, c.max_lds, halo, &t)) continue;
+        if (!tile_geometry(W, H, G, k->cpt, k->ntmax, k->pitch, c.max_lds, halo, &t)) continue;
         KernelChoice kc;
         kc.k = k;
         kc.tile = t;
         kc.block = (int)(((long)t.tw * (t.th / k->cpt) + 63) / 64 * 64);
-        kc.lds = (size_t)G * (t.tw + 2) * (t.th + 2) * sizeof(float);
+        kc.lds = (size_t)G * ((k->pitch > 0 ? k->pitch : t.tw) + 2) * (t.th + 2) * sizeof(float);
         kc.slot_floats = 4L * G * W * H;
         out.push_back(kc);
       }

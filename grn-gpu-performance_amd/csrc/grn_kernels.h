@@ -14,6 +14,8 @@ struct GrnKernelEntry {
   int ghi;    // largest gene count handled
   bool guard; // applies the diffusion guard (needed for fields that can go negative)
   const void *fn;
+  int pitch;  // halo-tiled: LDS window rows have the compile-time pitch pitch + 2, so the
+              // window is at most pitch cells wide; 0 = the window width + 2 (runtime)
 };
 
 // Instantiations of the on-chip field kernel, one table per translation unit.

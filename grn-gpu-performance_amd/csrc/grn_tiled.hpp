@@ -37,7 +37,12 @@ __device__ __noinline__ double score_terms_split(const float *hi, const float *l
   return acc;
 }
 
-template <int G, int CPT, int NTMAX, bool GUARD>
+// WP > 0: the LDS window rows have the compile-time pitch WP + 2 (the window
+// is at most WP cells wide), so every neighbour read has an immediate offset.
+// FAST: the program's state provably stays >= 0 (GrnProgram::fast), so the
+// step drops the minConc clamp and the log2 floor; the kernel switches once
+// per model between the two bodies.
+template <int G, int CPT, int NTMAX, bool GUARD, int WP, bool FAST>
 __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnProgram *__restrict__ progs,
                                                 const GrnDataset *__restrict__ datasets, int m, int pi,
                                                 Shared sh) {
@@ -51,12 +56,12 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   float *slot = a.scratch + (size_t)blockIdx.x * a.slot_floats;
   float *Ahi = slot, *Alo = slot + (size_t)G * N, *Bhi = slot + (size_t)2 * G * N, *Blo = slot + (size_t)3 * G * N;
 
-  Model<0, G, CPT, NTMAX, GUARD, true> f;
+  Model<WP, G, CPT, NTMAX, GUARD, true> f;
   f.P = P;
   f.sh = sh;
   f.W = TW;
   f.H = TH;
-  f.plane = (TW + 2) * (TH + 2);
+  f.plane = f.pc() * (TH + 2);
   f.dt = P->dt;
   f.min_conc = D->min_conc;
   f.clamp_any_r = !(D->min_conc > 0.f || !D->nonneg);
@@ -64,7 +69,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   f.active = tid < n_active;
   f.x = f.active ? tid % TW : 0;
   f.y0 = f.active ? (tid / TW) * CPT : 0;
-  f.ubase = f.y0 * (TW + 2) + f.x;
+  f.ubase = f.y0 * f.pc() + f.x;
   const int n_exp = D->n_exp, n_in = D->n_in, n_tgt = D->n_tgt;
   const int S = (a.step_cap > 0 && a.step_cap < D->steps) ? a.step_cap : D->steps;
   const double mcc = D->min_conc_change_d;
@@ -129,14 +134,14 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
           // last plane read, so the planes can be stored at once)
           f.store_planes();
           __syncthreads();
-          // (one step variant, with the per-gene diffusion branch: a second
-          // copy of the step loop in these kernels costs register spills)
+          // (the step with the per-gene diffusion branch only: a second
+          // copy of the step loop inside one body costs register spills)
           float c = 0.f;
           for (int t = 0; t < Tb; ++t) {
             if (track && t == Tb - 1)
-              c = f.template step<true, false, true>();
+              c = f.template step<true, FAST, true>();
             else
-              f.template step<false, false, true>();
+              f.template step<false, FAST, true>();
           }
           cblk = __builtin_fmaxf(cblk, c);
 #pragma unroll
@@ -198,19 +203,24 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   }
 }
 
-template <int CPT, int NTMAX, bool GUARD, int G, int GHI>
-__device__ __forceinline__ void dispatch_tiled(int g, const GrnLaunch &a, const GrnProgram *__restrict__ progs,
+template <int CPT, int NTMAX, bool GUARD, int WP, int G, int GHI>
+__device__ __forceinline__ void dispatch_tiled(int g, int fast, const GrnLaunch &a,
+                                               const GrnProgram *__restrict__ progs,
                                                const GrnDataset *__restrict__ datasets, int m, int pi,
                                                Shared sh) {
   if constexpr (G <= GHI) {
-    if (g == G)
-      run_tiled_model<G, CPT, NTMAX, GUARD>(a, progs, datasets, m, pi, sh);
-    else
-      dispatch_tiled<CPT, NTMAX, GUARD, G + 1, GHI>(g, a, progs, datasets, m, pi, sh);
+    if (g == G) {
+      if (fast)
+        run_tiled_model<G, CPT, NTMAX, GUARD, WP, true>(a, progs, datasets, m, pi, sh);
+      else
+        run_tiled_model<G, CPT, NTMAX, GUARD, WP, false>(a, progs, datasets, m, pi, sh);
+    } else {
+      dispatch_tiled<CPT, NTMAX, GUARD, WP, G + 1, GHI>(g, fast, a, progs, datasets, m, pi, sh);
+    }
   }
 }
 
-template <int CPT, int NTMAX, int GLO, int GHI, bool GUARD>
+template <int CPT, int NTMAX, int GLO, int GHI, bool GUARD, int WP>
 __global__ __launch_bounds__(NTMAX) void tiled_kernel(GrnLaunch a, const GrnProgram *__restrict__ progs,
                                                       const GrnDataset *__restrict__ datasets,
                                                       const int *__restrict__ order) {
@@ -230,12 +240,13 @@ __global__ __launch_bounds__(NTMAX) void tiled_kernel(GrnLaunch a, const GrnProg
     if (m >= n) break;
     const int pi = __builtin_amdgcn_readfirstlane(ord[m]);
     const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
-    dispatch_tiled<CPT, NTMAX, GUARD, GLO, GHI>(g, a, progs, datasets, m, pi, sh);
+    const int fast = __builtin_amdgcn_readfirstlane(progs[pi].fast);
+    dispatch_tiled<CPT, NTMAX, GUARD, WP, GLO, GHI>(g, fast, a, progs, datasets, m, pi, sh);
     __syncthreads();
   }
 }
 
 }  // namespace grn_field
 
-#define GRN_TILED_ENTRY(CPT, NTMAX, GLO, GHI, GUARD) \
-  {-1, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::tiled_kernel<CPT, NTMAX, GLO, GHI, GUARD>}
+#define GRN_TILED_ENTRY(CPT, NTMAX, GLO, GHI, GUARD, WP) \
+  {-1, CPT, NTMAX, GLO, GHI, GUARD, (const void *)&grn_field::tiled_kernel<CPT, NTMAX, GLO, GHI, GUARD, WP>, WP}
