@@ -319,7 +319,7 @@ std::vector<KernelChoice> kernel_candidates(const Ctx &c, int W, int H, int G, i
         kc.tile = t;
         kc.block = (int)(((long)t.tw * (t.th / k->cpt) + 63) / 64 * 64);
         kc.lds = (size_t)G * ((k->pitch > 0 ? k->pitch : t.tw) + 2) * (t.th + 2) * sizeof(float);
-        kc.slot_floats = 4L * G * W * H;
+        kc.slot_floats = 2L * grn_cell_rec(G) * W * H;  // two sets of per-cell (hi, lo) records
         out.push_back(kc);
       }
     }

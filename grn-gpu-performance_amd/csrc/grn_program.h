@@ -119,6 +119,12 @@ struct GrnBatch {
   GrnQueue q[GRN_MAX_QUEUES];
 };
 
+// Halo-tiled kernels keep a model's state in device memory as one record of
+// grn_cell_rec(G) floats per cell: hi[0..G), lo[0..G), zero padding to a
+// multiple of four floats (16-byte loads and stores).  (constexpr: callable
+// from host and device code.)
+constexpr int grn_cell_rec(int G) { return (2 * G + 3) & ~3; }
+
 struct GrnLaunch {
   const GrnProgram *progs;  // [n_total]
   const GrnProgram64 *progs64;  // [n_total] (FP64 kernel only; entries of other models unset)

@@ -2,8 +2,9 @@
 // LDS of one workgroup (e.g. 96x96, 100x100, 128x128 with many genes).
 //
 // One workgroup still owns one model at a time (persistent, work queue).  The
-// model's state lives in device memory as two (hi, lo) plane sets, A (time s)
-// and B (time s + T).  The grid is covered by tiles of tile_sx x tile_sy
+// model's state lives in device memory as two sets of per-cell records
+// (hi[0..G), lo[0..G), padding: grn_cell_rec), A (time s) and B (time s + T),
+// so a window cell's whole state moves with a few 16-byte loads.  The grid is covered by tiles of tile_sx x tile_sy
 // cells; each tile is processed in an extended window of tile_w x tile_h cells
 // (a halo of tile_halo = T cells on every side):
 //   load the window from A into registers and the LDS mirror, run T Euler
@@ -21,12 +22,17 @@
 
 namespace grn_field {
 
-// Fitness terms straight from the device-memory state (c = hi + lo).
-__device__ __noinline__ double score_terms_split(const float *hi, const float *lo, const double *target,
-                                                 int count, double thr, int mse) {
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// Fitness terms straight from the device-memory state (c = hi + lo) of the
+// target genes n_in .. n_in + count / N - 1, in [target][cell] order.
+__device__ __noinline__ double score_terms_rec(const float *A, int rec, int G, int n_in, int N,
+                                               const double *target, int count, double thr, int mse) {
   double acc = 0.0;
   for (int i = threadIdx.x; i < count; i += blockDim.x) {
-    const double d = ((double)hi[i] + (double)lo[i]) - target[i];
+    const int tg = i / N, cell = i - tg * N;
+    const float *r = A + (size_t)cell * rec + n_in + tg;
+    const double d = ((double)r[0] + (double)r[G]) - target[i];
     if (mse) {
       acc += d * d;
     } else {
@@ -53,8 +59,9 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   const int W = D->W, H = D->H, N = W * H;
   const int TW = a.tile_w, TH = a.tile_h, T = a.tile_halo, SX = a.tile_sx, SY = a.tile_sy;
   const int ntx = (W + SX - 1) / SX, nty = (H + SY - 1) / SY;
-  float *slot = a.scratch + (size_t)blockIdx.x * a.slot_floats;
-  float *Ahi = slot, *Alo = slot + (size_t)G * N, *Bhi = slot + (size_t)2 * G * N, *Blo = slot + (size_t)3 * G * N;
+  constexpr int R = grn_cell_rec(G);  // floats per cell record
+  float *A = a.scratch + (size_t)blockIdx.x * a.slot_floats;  // state at time s: N records
+  float *B = A + (size_t)R * N;                               // state at time s + T
 
   Model<WP, G, CPT, NTMAX, GUARD, true> f;
   f.P = P;
@@ -89,12 +96,20 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
   int e = 0, steps_total = 0;
   while (e < n_exp && err <= max_err) {
     // Initial state at time 0 in A: inputs for genes < nIn, zero elsewhere.
-    for (int i = tid; i < G * N; i += blockDim.x) {
-      const int g = i / N;
-      const double v = g < n_in ? D->init[(size_t)e * n_in * N + i] : 0.0;
-      const float h = (float)v;
-      Ahi[i] = h;
-      Alo[i] = (float)(v - (double)h);
+    for (int cell = tid; cell < N; cell += blockDim.x) {
+      float r[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) r[j] = 0.f;
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g < n_in) {
+          const double v = D->init[((size_t)e * n_in + g) * N + cell];
+          r[g] = (float)v;
+          r[G + g] = (float)(v - (double)r[g]);
+        }
+      float4v *dst = reinterpret_cast<float4v *>(A + (size_t)cell * R);
+#pragma unroll
+      for (int j = 0; j < R / 4; ++j) dst[j] = float4v{r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]};
     }
     __syncthreads();
 
@@ -120,16 +135,25 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
             gidx[k] = in ? gy * W + gx : -1;
           }
 #pragma unroll
-          for (int g = 0; g < G; ++g)
+          for (int k = 0; k < CPT; ++k) {
+            // load from an in-range record, then select: no out-of-range
+            // address is ever formed for cells outside the grid
+            const float4v *src = reinterpret_cast<const float4v *>(A + (size_t)(gidx[k] >= 0 ? gidx[k] : 0) * R);
+            float r[R];
 #pragma unroll
-            for (int k = 0; k < CPT; ++k) {
-              // load from an in-range index, then select: no out-of-range
-              // address is ever formed for cells outside the grid
-              const size_t i = (size_t)g * N + (gidx[k] >= 0 ? gidx[k] : 0);
-              const float vh = Ahi[i], vl = Alo[i];
-              f.hi[g][k] = gidx[k] >= 0 ? vh : 0.f;
-              f.lo[g][k] = gidx[k] >= 0 ? vl : 0.f;
+            for (int j = 0; j < R / 4; ++j) {
+              const float4v q = src[j];
+              r[4 * j] = q.x;
+              r[4 * j + 1] = q.y;
+              r[4 * j + 2] = q.z;
+              r[4 * j + 3] = q.w;
             }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+              f.hi[g][k] = gidx[k] >= 0 ? r[g] : 0.f;
+              f.lo[g][k] = gidx[k] >= 0 ? r[G + g] : 0.f;
+            }
+          }
           // (the previous tile's last step ended with a barrier after its
           // last plane read, so the planes can be stored at once)
           f.store_planes();
@@ -146,19 +170,24 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
           cblk = __builtin_fmaxf(cblk, c);
 #pragma unroll
           for (int k = 0; k < CPT; ++k)
-            if (f.trk[k] != 0.f)
+            if (f.trk[k] != 0.f) {
+              float r[R];
 #pragma unroll
               for (int g = 0; g < G; ++g) {
-                Bhi[(size_t)g * N + gidx[k]] = f.hi[g][k];
-                Blo[(size_t)g * N + gidx[k]] = f.lo[g][k];
+                r[g] = f.hi[g][k];
+                r[G + g] = f.lo[g][k];
               }
+#pragma unroll
+              for (int j = 2 * G; j < R; ++j) r[j] = 0.f;
+              float4v *dst = reinterpret_cast<float4v *>(B + (size_t)gidx[k] * R);
+#pragma unroll
+              for (int j = 0; j < R / 4; ++j) dst[j] = float4v{r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]};
+            }
         }
       }
-      float *t0 = Ahi, *t1 = Alo;
-      Ahi = Bhi;
-      Alo = Blo;
-      Bhi = t0;
-      Blo = t1;
+      float *t = A;
+      A = B;
+      B = t;
       __syncthreads();  // the new state is complete before the next block reads it
       s += Tb;
       if (track) chg = cblk;
@@ -167,8 +196,8 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
     chg_acc += (double)chg;
 
     const bool mse = D->kind == 1;
-    double acc = score_terms_split(Ahi + (size_t)n_in * N, Alo + (size_t)n_in * N,
-                                   D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr, mse ? 1 : 0);
+    double acc = score_terms_rec(A, R, G, n_in, N, D->target + (size_t)e * n_tgt * N, n_tgt * N, D->thr,
+                                 mse ? 1 : 0);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) sh.wsum[wave] = acc;
@@ -185,7 +214,10 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
     err += e_err / n_exp;
     if (a.fields && e == a.field_exp) {
       double *fo = a.fields + (size_t)m * a.field_stride;
-      for (int i = tid; i < G * N; i += blockDim.x) fo[i] = (double)Ahi[i] + (double)Alo[i];
+      for (int i = tid; i < G * N; i += blockDim.x) {
+        const int g = i / N, cell = i - g * N;
+        fo[i] = (double)A[(size_t)cell * R + g] + (double)A[(size_t)cell * R + G + g];
+      }
     }
     ++e;
     __syncthreads();  // wsum reads and the state reads done before reuse
