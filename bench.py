@@ -104,6 +104,52 @@ def cpu_baseline(w, gpu_res=None, budget_s=22.0):
             "parity_vs_gpu": parity}
 
 
+def config0_full(ctx, seed=0):
+    """The reference's own CPU-runnable case in full (64 models, 3 genes,
+    32x32, 1000 steps): the FP64 oracle on one thread and on the host share
+    of cores, the device on the same models (one batch, and one model per
+    launch), and the device's results against the oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import grn
+    import grn_oracle as ora
+    import workloads
+
+    w = workloads.config0(seed=seed)
+    workloads.make_target(ctx, w)
+    ds = ctx.add_dataset(w.params, w.init, w.target)
+    p = ora.from_simparams(w.params)
+    cells = w.params.dim_x * w.params.dim_y
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    r1 = ora.evaluate_batch(p, w.init, w.target, w.models, n_threads=1)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rn = ora.evaluate_batch(p, w.init, w.target, w.models, n_threads=threads)
+    tn = time.perf_counter() - t0
+    cs = float(np.sum(r1["steps"]) * cells)
+    g = ctx.evaluate_batch(w.models, datasets=ds)
+    t0 = time.perf_counter()
+    g = ctx.evaluate_batch(w.models, datasets=ds)
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for m in w.models:
+        ctx.evaluate(m, dataset=ds)
+    tp = time.perf_counter() - t0
+    rel = np.abs(g["raw_error"] - r1["raw_error"]) / np.abs(r1["raw_error"])
+    return {"workload": "config 0: 64 models, 3 genes, 2-6 links, 32x32, 1000 steps, dt 0.01, MSE",
+            "cell_steps": cs, "cpu_1thread_s": t1, "cpu_1thread_cell_steps_per_s": cs / t1,
+            "cpu_threads": threads, "cpu_host_cpus": os.cpu_count(), "cpu_threads_s": tn,
+            "cpu_threads_cell_steps_per_s": cs / tn,
+            "gpu_batched_s": tb, "gpu_batched_cell_steps_per_s": cs / tb,
+            "gpu_plain_s": tp, "gpu_plain_cell_steps_per_s": cs / tp,
+            "parity_vs_oracle": {"max_rel_fitness": float(rel.max()),
+                                 "steps_equal": bool(np.array_equal(g["steps"], r1["steps"])),
+                                 "ranking_identical": bool(np.array_equal(
+                                     np.argsort(g["raw_error"], kind="stable"),
+                                     np.argsort(r1["raw_error"], kind="stable"))),
+                                 "threads_bitwise_equal": bool(np.array_equal(rn["raw_error"], r1["raw_error"]))}}
+
+
 def measure_islands(ctx, seed, islands, island_size, warmup, steps, tune=True, tune_sample=256, tune_steps=100):
     """Multi-island evolution generations (16 islands x 1024 models, sides
     32-128, 3000 steps): each generation is the simulation and fitness of
@@ -164,7 +210,8 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     """Generations of the full-evolution workload at its real grid (100x100,
     5000 steps, four inputs, hidden 10-gene target): n_islands x island_size
     models per generation, plain or screened by the learned surrogate (4
-    candidates per slot, the better-predicted half kept).  Returns a dict."""
+    candidates per slot, the best-predicted one simulated; the surrogate
+    trained on (child, slot parent) pairs).  Returns a dict."""
     import grn
     import evolve
 
@@ -175,23 +222,24 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     sur = None
     if screened:
         sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], seed=prob.search["seed"])
+        sur.set_pair_weight(4.0)
         search.set_oversample(4)
-        search.set_screen_policy(grn.SCREEN_HALF)
+        search.set_screen_policy(grn.SCREEN_ABSOLUTE)
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     # generation 0 (the initial population) is simulated unscreened
     log(f"[bench] config 4: {n_islands} x {island_size} models, {'screened' if screened else 'plain'}, generation 0")
     t0 = time.perf_counter()
-    g0 = search.run_generation_screened(ctx, sur, train_steps=10)[0] if screened else search.run_generation(ctx)
+    g0 = search.run_generation_screened(ctx, sur, train_steps=20)[0] if screened else search.run_generation(ctx)
     gen0_s = time.perf_counter() - t0
     st0 = ctx.stats()
     for _ in range(warmup):
-        search.run_generation_screened(ctx, sur, train_steps=10) if screened else search.run_generation(ctx)
+        search.run_generation_screened(ctx, sur, train_steps=20) if screened else search.run_generation(ctx)
     cell_steps, kernel_ms = 0.0, 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         log(f"[bench] config 4: timed generation {k + 1}")
         if screened:
-            g, sc = search.run_generation_screened(ctx, sur, train_steps=10)
+            g, sc = search.run_generation_screened(ctx, sur, train_steps=20)
         else:
             g = search.run_generation(ctx)
         st = ctx.stats()
@@ -302,7 +350,8 @@ def bench_time_to_fitness(args):
             runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-4,
                                        oversample=args.oversample, train_steps=args.train_steps,
                                        screen_fraction=args.screen_fraction, screen_policy=args.screen_policy,
-                                       screen=args.screen, coarse_side=args.coarse_side, progress=log)
+                                       screen=args.screen, coarse_side=args.coarse_side,
+                                       pair_weight=args.pair_weight if screened else 0.0, progress=log)
         plain, scr = runs["plain"], runs["screened"]
         thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
         table = []
@@ -345,7 +394,8 @@ def bench_time_to_fitness(args):
                    "screen_policy": ["absolute", "delta", "better-half"][args.screen_policy],
                    "screen": args.screen if args.screen == "mlp" else
                    f"coarse simulation on {args.coarse_side}x{args.coarse_side}",
-                   "train_steps_per_generation": args.train_steps if args.screen == "mlp" else None},
+                   "train_steps_per_generation": args.train_steps if args.screen == "mlp" else None,
+                   "pair_weight": args.pair_weight if args.screen == "mlp" else None},
         "speedup_per_seed": ratios,
         "ms_per_generation": {k: float(np.mean(v)) for k, v in gen_ms.items()},
         "screened_phase_ms": {k: float(np.mean(v)) for k, v in phase.items() if v},
@@ -418,7 +468,7 @@ def main():
     ap.add_argument("--models", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--seeds", default="0,1,2,3,4,5,6,7", help="evolve: comma-separated problem/search seeds")
-    ap.add_argument("--screen-policy", type=int, default=2, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
+    ap.add_argument("--screen-policy", type=int, default=0, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
     ap.add_argument("--screen", choices=["mlp", "coarse"], default="mlp",
                     help="evolve: score candidates with the learned surrogate or a coarse-grid simulation")
     ap.add_argument("--coarse-side", type=int, default=8, help="evolve --screen coarse: side of the coarse grid")
@@ -437,7 +487,9 @@ def main():
     ap.add_argument("--oversample", type=int, default=4)
     ap.add_argument("--screen-fraction", type=float, default=1.0,
                     help="evolve: fraction of each island's slots simulated per generation")
-    ap.add_argument("--train-steps", type=int, default=10)
+    ap.add_argument("--train-steps", type=int, default=20)
+    ap.add_argument("--pair-weight", type=float, default=4.0,
+                    help="evolve: > 0 trains the surrogate on (child, slot parent) pairs as well")
     ap.add_argument("--no-tune", action="store_true", help="config2: skip the autotuner")
     ap.add_argument("--tune", action="store_true", help="config1: autotune the size classes first")
     ap.add_argument("--tune-sample", type=int, default=256)
@@ -531,6 +583,8 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline")
         cpu = cpu_baseline(w, gpu_res=res)
+        log("[bench] config 0 in full on the CPU oracle and the device")
+        cpu["config0_full"] = config0_full(ctx, args.seed)
 
     # the other single-GPU workloads at their real sizes (reported beside the
     # headline, not in `value`)

@@ -36,9 +36,6 @@
 
 namespace grn_field {
 
-// Per-cell log2 values, indexed by a uniform regulator index.  Sixteen lanes
-// keep the vector above the size at which the compiler turns a dynamic index
-// into a select chain, so each read is one indexed register move.
 // Per-cell log2 values, indexed by a uniform regulator index.  Built with
 // -amdgpu-use-divergent-register-indexing so that every width is read with
 // one indexed register move (s_set_gpr_idx) rather than a select chain.
@@ -85,14 +82,36 @@ struct Model {
   bool clamp_any_r;      // the R != 0 test of the clamp is moot (min_conc <= 0, field >= 0)
   bool active;
   int x, y0, ubase;      // ubase: LDS index of the cell north-west of cell k=0
-  float nf[CPT];         // neighbour count per cell
-  float dtc[TILED ? CPT : 1];  // per-cell time step (TILED)
-  float trk[TILED ? CPT : 1];  // 1 where the cell's |R| counts (TILED)
+  float nf[TILED ? 1 : CPT];  // neighbour count per cell (on-chip kernels)
+  // TILED: the per-cell constants of the current window packed into two
+  // registers instead of 3 CPT: nfb holds each cell's neighbour count in 3
+  // bits (cell k at bit 3k); cellb bit k = the cell lies in the grid (time
+  // step dt, else 0), bit 16 + k = its |R| counts towards the max change.
+  unsigned nfb, cellb;
   float hi[G][CPT];
   float lo[G][CPT];
   Shared sh;
+  static_assert(!TILED || CPT <= 10, "the packed per-cell constants hold at most 10 cells");
 
   __device__ __forceinline__ int pc() const { return WC ? WC + 2 : W + 2; }
+  __device__ __forceinline__ float nfv(int k) const {
+    if constexpr (TILED)
+      return (float)((nfb >> (3 * k)) & 7u);
+    else
+      return nf[k];
+  }
+  __device__ __forceinline__ float dtv(int k) const {
+    if constexpr (TILED)
+      return ((cellb >> k) & 1u) ? dt : 0.f;
+    else
+      return dt;
+  }
+  __device__ __forceinline__ bool trkv(int k) const {
+    if constexpr (TILED)
+      return ((cellb >> (16 + k)) & 1u) != 0;
+    else
+      return true;
+  }
 
   // Only diffusing genes are read back from the LDS mirror.
   __device__ __forceinline__ void store_planes() {
@@ -142,16 +161,16 @@ struct Model {
         const float up = pl[k * pc() + 1];
         const float down = pl[(k + 2) * pc() + 1];
         const float S = ((left + right) + up) + down;
-        float lap = __builtin_fmaf(-nf[k], c, S);
+        float lap = __builtin_fmaf(-nfv(k), c, S);
         if (GUARD) {  // modelsimulator.cpp:255; moot for a nonnegative field
-          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+          const bool use = (S > 0.f) || (nfv(k) > 0.f && c > 0.f);
           lap = use ? lap : 0.f;
         }
         R = __builtin_fmaf(gp.dif, lap, R);
       }
       // Compensated Euler update c += dt*R (Fast2Sum on the pair) with
       // the minConc clamp of modelsimulator.cpp:263-275.
-      const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
+      const float yv = __builtin_fmaf(dtv(k), R, lo[p][k]);
       const float sv = c + yv;
       const float nl = yv - (sv - c);
       if (FAST) {
@@ -162,7 +181,7 @@ struct Model {
         hi[p][k] = clamp ? 0.f : sv;
         lo[p][k] = clamp ? 0.f : nl;
       }
-      if (TRACK) *cmax = __builtin_fmaxf(*cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
+      if (TRACK) *cmax = __builtin_fmaxf(*cmax, trkv(k) ? __builtin_fabsf(R) : 0.f);
     }
   }
   template <bool TRACK, bool FAST, bool DIF>
@@ -189,9 +208,9 @@ struct Model {
         const float up = pl[k * pc() + 1];
         const float down = pl[(k + 2) * pc() + 1];
         const float S = ((left + right) + up) + down;
-        float lap = __builtin_fmaf(-nf[k], c, S);
+        float lap = __builtin_fmaf(-nfv(k), c, S);
         if (GUARD) {
-          const bool use = (S > 0.f) || (nf[k] > 0.f && c > 0.f);
+          const bool use = (S > 0.f) || (nfv(k) > 0.f && c > 0.f);
           lap = use ? lap : 0.f;
         }
         Rv[j] = __builtin_fmaf(gp.dif, lap, Rv[j]);
@@ -202,7 +221,7 @@ struct Model {
       const int k = k0 + j;
       const float c = hi[p][k];
       const float R = Rv[j];
-      const float yv = __builtin_fmaf(TILED ? dtc[k] : dt, R, lo[p][k]);
+      const float yv = __builtin_fmaf(dtv(k), R, lo[p][k]);
       const float sv = c + yv;
       const float nl = yv - (sv - c);
       if (FAST) {
@@ -213,7 +232,7 @@ struct Model {
         hi[p][k] = clamp ? 0.f : sv;
         lo[p][k] = clamp ? 0.f : nl;
       }
-      if (TRACK) *cmax = __builtin_fmaxf(*cmax, (!TILED || trk[k] != 0.f) ? __builtin_fabsf(R) : 0.f);
+      if (TRACK) *cmax = __builtin_fmaxf(*cmax, trkv(k) ? __builtin_fabsf(R) : 0.f);
     }
   }
 
@@ -233,6 +252,9 @@ struct Model {
       // Reload the program through an opaque offset in every block, so its
       // constants are not kept in scalar registers across blocks.
       const GrnProgram *Pq = P + opaque_zero();
+      // (tiled: the packed per-cell constants are unpacked per block, not
+      // hoisted out of the step loop into one register per cell)
+      if constexpr (TILED) asm volatile("" : "+v"(nfb), "+v"(cellb));
       // log2 of every gene of this block's cells (their step-start values:
       // a cell's update touches only that cell)
       lg_t lg[CB];

@@ -29,6 +29,8 @@ extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wanyguard(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_tiled512s(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_tiled512g(int *n);
 // The FP64 kernel (grn_exact.hip): wc = -2.
 extern "C" const GrnKernelEntry *grn_field_table_exact(int *n);
 

@@ -59,8 +59,10 @@ struct MlpAdam {
 
 hipError_t grn_mlp_gemm(int epi, const MlpGemm &g, hipStream_t s);
 hipError_t grn_mlp_out_forward(const __bf16 *H, const __bf16 *w, const float *b0, float *y, int rows, hipStream_t s);
+// pair_w > 0: rows come in (child, parent) pairs and the loss adds the
+// pairwise difference term (rows must then be even).
 hipError_t grn_mlp_out_backward(const float *y, const float *t, const float *mask, const __bf16 *w, const float *Z,
-                                float inv_n, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
+                                float inv_n, float pair_w, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
                                 hipStream_t s);
 hipError_t grn_mlp_row_dot(const __bf16 *X, const float *v, int rows, int cols, float *out, hipStream_t s);
 hipError_t grn_mlp_sum(const float *x, int n, float *out, hipStream_t s);

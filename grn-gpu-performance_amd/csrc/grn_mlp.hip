@@ -174,18 +174,27 @@ __global__ __launch_bounds__(256) void out_forward_kernel(const bf16 *__restrict
 // Loss gradient at the output and the last hidden layer's dZ:
 //   dy[b] = mask[b] * 2 (y[b] - t[b]) / n_valid,
 //   dZ[b][k] = dy[b] * w[k] * gelu'(Z[b][k]), written as dZ and dZ^T.
+// pair_w > 0: rows 2i and 2i + 1 are a pair (a child and its parent), and
+// the loss adds pair_w * ((y_2i - y_2i+1) - (t_2i - t_2i+1))^2 per pair, so
+// dy[b] gains 2 pair_w e_b / n_valid with e_b the pair's difference error
+// seen from row b (the partner's is -e_b).
 __global__ __launch_bounds__(256) void out_backward_kernel(const float *__restrict__ y, const float *__restrict__ tgt,
                                                            const float *__restrict__ mask, const bf16 *__restrict__ w,
-                                                           const float *__restrict__ Z, float inv_n, int rows,
-                                                           float *__restrict__ dy, float *__restrict__ sq,
+                                                           const float *__restrict__ Z, float inv_n, float pair_w,
+                                                           int rows, float *__restrict__ dy, float *__restrict__ sq,
                                                            bf16 *__restrict__ dZ, bf16 *__restrict__ dZt) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= rows) return;
   const float e = mask[row] * (y[row] - tgt[row]);
-  const float d = 2.f * e * inv_n;
+  float ep = 0.f;
+  if (pair_w > 0.f) {
+    const int q = row ^ 1;
+    ep = mask[row] * mask[q] * ((y[row] - y[q]) - (tgt[row] - tgt[q]));
+  }
+  const float d = 2.f * (e + pair_w * ep) * inv_n;
   if (lane == 0) {
     dy[row] = d;
-    sq[row] = e * e;
+    sq[row] = e * e + 0.5f * pair_w * ep * ep;  // each pair's term is shared by its two rows
   }
   const bf16x8 ww = *(const bf16x8 *)(w + 8 * lane);
 #pragma unroll
@@ -304,10 +313,11 @@ hipError_t grn_mlp_out_forward(const __bf16 *H, const __bf16 *w, const float *b0
 }
 
 hipError_t grn_mlp_out_backward(const float *y, const float *t, const float *mask, const __bf16 *w, const float *Z,
-                                float inv_n, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
+                                float inv_n, float pair_w, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
                                 hipStream_t s) {
-  hipLaunchKernelGGL(out_backward_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, y, t, mask, w, Z, inv_n, rows, dy,
-                     sq, dZ, dZt);
+  if (pair_w > 0.f && rows % 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(out_backward_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, y, t, mask, w, Z, inv_n, pair_w,
+                     rows, dy, sq, dZ, dZt);
   return hipGetLastError();
 }
 

@@ -626,6 +626,16 @@ void Search::screen(const double *scores, std::vector<size_t> *chosen) {
   rows_written_ = false;
 }
 
+const Individual *Search::slot_parent(const Individual *child) const {
+  for (int is = 0; is < p_.n_islands; ++is) {
+    const std::vector<Individual> &kids = children_[is];
+    if (kids.empty() || child < kids.data() || child >= kids.data() + kids.size()) continue;
+    const size_t i = (size_t)(child - kids.data());
+    return i < perm_[is].size() ? &islands_[is][perm_[is][i]] : nullptr;
+  }
+  return nullptr;
+}
+
 void Search::choose_next_generation(int is, std::vector<Individual *> *winners) {
   // SearchAlgoDetCrowd::chooseNextGeneration (searchalgodetcrowd.cpp:103-147):
   // each child competes with its own parent and wins ties
@@ -1177,7 +1187,34 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   }
   // learn from the simulated children (their rows are candidate rows
   // already encoded, except for the initial population), then select and breed
-  {
+  if (S->pair_weight() > 0 && !delta) {
+    // pair mode: every child with its slot parent (the initial population
+    // as pairs of a model with itself)
+    std::vector<int> idx;
+    std::vector<double> tc, tp;
+    std::vector<const grn::Model *> cm, pm;
+    for (int i = 0; i < n; ++i) {
+      if (std::isnan(target[i])) continue;
+      const grn::Individual *child = h->s->pending()[i];
+      const grn::Individual *par = cand_rows ? h->s->slot_parent(child) : nullptr;
+      const bool paired = par && !std::isnan(par->log_raw);
+      idx.push_back(i);
+      tc.push_back(target[i]);
+      tp.push_back(paired ? par->log_raw : target[i]);
+      cm.push_back(&child->model);
+      pm.push_back(paired ? &par->model : &child->model);
+    }
+    std::vector<uint16_t> crow(idx.size() * (size_t)MLP_WIDTH), prow(idx.size() * (size_t)MLP_WIDTH);
+    if (cand_rows) {
+      for (size_t j = 0; j < idx.size(); ++j)
+        std::memcpy(crow.data() + j * MLP_WIDTH, cand_rows + chosen[idx[j]] * MLP_WIDTH,
+                    sizeof(uint16_t) * MLP_WIDTH);
+    } else {
+      S->encode_rows(cm, crow.data());
+    }
+    S->encode_rows(pm, prow.data());
+    if ((rc = S->add_pairs((int)idx.size(), crow.data(), prow.data(), tc.data(), tp.data()))) return rc;
+  } else {
     std::vector<const grn::Model *> ms;
     std::vector<int> idx;
     std::vector<double> logs;

@@ -114,6 +114,10 @@ class Search {
   // Writes every candidate's row as breeding would (rows: candidates() order).
   void encode_candidates(uint16_t *rows, int row_width);
 
+  // The parent a pending child competes with (its slot's parent), or null
+  // for a member of the initial population.  Valid until advance().
+  const Individual *slot_parent(const Individual *child) const;
+
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
   const std::vector<std::vector<Individual>> &islands() const { return islands_; }

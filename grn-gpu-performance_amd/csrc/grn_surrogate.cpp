@@ -243,18 +243,52 @@ int Surrogate::add(int n, const float *features, const double *log_errors) {
 }
 
 int Surrogate::add_rows(int n, const uint16_t *rows, const double *log_errors) {
+  if (pair_w_ > 0.f) return add_pairs(n, rows, rows, log_errors, log_errors);
   if (int rc0 = finish()) return rc0;
   if (n <= 0) return 0;
   std::vector<float> t(n);
   for (int i = 0; i < n; ++i) t[i] = (float)log_errors[i];
+  return store_rows(n, rows, t.data());
+}
+
+int Surrogate::set_pair_weight(double w) {
+  if (!(w >= 0.0) || !std::isfinite(w)) return set_last_error(GRN_E_INVALID, "pair weight %g", w);
+  if (n_samples_ > 0 && (w > 0.0) != (pair_w_ > 0.f))
+    return set_last_error(GRN_E_INVALID, "the store's layout (pairs or single rows) is fixed once it holds samples");
+  if (w > 0.0 && p_.capacity % 2)
+    return set_last_error(GRN_E_INVALID, "pair mode needs an even store capacity (%ld)", p_.capacity);
+  pair_w_ = (float)w;
+  return 0;
+}
+
+int Surrogate::add_pairs(int n, const uint16_t *child_rows, const uint16_t *parent_rows,
+                         const double *child_log_errors, const double *parent_log_errors) {
+  if (pair_w_ <= 0.f) return set_last_error(GRN_E_INVALID, "pairs need a positive pair weight");
+  if (int rc0 = finish()) return rc0;
+  if (n <= 0) return 0;
+  // interleaved: child 2i, parent 2i + 1 (the store and its capacity stay
+  // pair-aligned, so a pair never straddles the ring's end)
+  std::vector<uint16_t> rows((size_t)2 * n * MLP_WIDTH);
+  std::vector<float> t((size_t)2 * n);
+  for (int i = 0; i < n; ++i) {
+    std::memcpy(rows.data() + (size_t)(2 * i) * MLP_WIDTH, child_rows + (size_t)i * MLP_WIDTH,
+                sizeof(uint16_t) * MLP_WIDTH);
+    std::memcpy(rows.data() + (size_t)(2 * i + 1) * MLP_WIDTH, parent_rows + (size_t)i * MLP_WIDTH,
+                sizeof(uint16_t) * MLP_WIDTH);
+    t[2 * i] = (float)child_log_errors[i];
+    t[2 * i + 1] = (float)parent_log_errors[i];
+  }
+  return store_rows(2 * n, rows.data(), t.data());
+}
+
+int Surrogate::store_rows(int n, const uint16_t *rows, const float *t) {
   if ((long)host_target_.size() < p_.capacity) host_target_.resize(std::min<long>(p_.capacity, n_samples_ + n));
   int done = 0;
   while (done < n) {
     const int k = (int)std::min<long>(n - done, p_.capacity - next_slot_);
     SUR_TRY(hipMemcpyAsync(d_feat_ + (size_t)next_slot_ * MLP_WIDTH, rows + (size_t)done * MLP_WIDTH,
                            sizeof(uint16_t) * (size_t)k * MLP_WIDTH, hipMemcpyHostToDevice, stream_));
-    SUR_TRY(hipMemcpyAsync(d_target_ + next_slot_, t.data() + done, sizeof(float) * k, hipMemcpyHostToDevice,
-                           stream_));
+    SUR_TRY(hipMemcpyAsync(d_target_ + next_slot_, t + done, sizeof(float) * k, hipMemcpyHostToDevice, stream_));
     for (int i = 0; i < k; ++i) host_target_[next_slot_ + i] = t[done + i];
     next_slot_ = (next_slot_ + k) % p_.capacity;
     done += k;
@@ -331,7 +365,16 @@ int Surrogate::train_async(int steps) {
   const int B = p_.batch;
   host_idx_.resize((size_t)steps * B);
   Rng rng(rng_);
-  for (int &v : host_idx_) v = (int)(rng.next() % (uint64_t)n_samples_);
+  if (pair_w_ > 0.f) {  // whole pairs: rows 2j and 2j + 1 of the store
+    const uint64_t n_pairs = (uint64_t)std::max<long>(n_samples_ / 2, 1);
+    for (size_t b = 0; b + 1 < host_idx_.size(); b += 2) {
+      const int j = (int)(rng.next() % n_pairs);
+      host_idx_[b] = 2 * j;
+      host_idx_[b + 1] = 2 * j + 1;
+    }
+  } else {
+    for (int &v : host_idx_) v = (int)(rng.next() % (uint64_t)n_samples_);
+  }
   rng_ = rng.next();
   if (host_idx_.size() > idx_cap_) {
     if (d_idx_) (void)hipFree(d_idx_);
@@ -347,7 +390,8 @@ int Surrogate::train_async(int steps) {
     SUR_TRY(grn_mlp_gather(d_feat_, d_target_, d_idx_ + (size_t)st * B, B, mu_, 1.f / sigma_, d_x_, d_xt_, d_t_,
                            stream_));
     if ((rc = forward(B, true))) return rc;
-    SUR_TRY(grn_mlp_out_backward(d_y_, d_t_, d_mask_, d_wout_, d_z_[MLP_HIDDEN_LAYERS - 1], 1.f / B, B, d_dy_, d_sq_,
+    SUR_TRY(grn_mlp_out_backward(d_y_, d_t_, d_mask_, d_wout_, d_z_[MLP_HIDDEN_LAYERS - 1], 1.f / B, pair_w_, B, d_dy_,
+                                 d_sq_,
                                  d_dz_[MLP_HIDDEN_LAYERS - 1], d_dzt_[MLP_HIDDEN_LAYERS - 1], stream_));
     SUR_TRY(grn_mlp_row_dot(d_ht_[MLP_HIDDEN_LAYERS - 1], d_dy_, MLP_WIDTH, B, d_g_ + MLP_OFF_OUT_W, stream_));
     SUR_TRY(grn_mlp_sum(d_dy_, B, d_g_ + MLP_OFF_OUT_B, stream_));
@@ -414,6 +458,7 @@ int Surrogate::gradient(int n, const float *features, const double *targets, flo
   if (int rc0 = finish()) return rc0;
   if (n < 1 || round_rows(n) > rows_cap_)
     return set_last_error(GRN_E_INVALID, "gradient takes 1 to %d rows", rows_cap_);
+  if (pair_w_ > 0.f && n % 2) return set_last_error(GRN_E_INVALID, "pair mode: rows come in pairs (%d rows)", n);
   const int rows = round_rows(n);
   std::vector<uint16_t> x, xt((size_t)rows * MLP_WIDTH);
   to_bf16_rows(n, rows, features, &x);
@@ -430,7 +475,7 @@ int Surrogate::gradient(int n, const float *features, const double *targets, flo
   SUR_TRY(hipMemcpyAsync(d_mask_, mask.data(), sizeof(float) * rows, hipMemcpyHostToDevice, stream_));
   int rc = forward(rows, true);
   if (!rc) {
-    SUR_TRY(grn_mlp_out_backward(d_y_, d_t_, d_mask_, d_wout_, d_z_[MLP_HIDDEN_LAYERS - 1], 1.f / n, rows, d_dy_,
+    SUR_TRY(grn_mlp_out_backward(d_y_, d_t_, d_mask_, d_wout_, d_z_[MLP_HIDDEN_LAYERS - 1], 1.f / n, pair_w_, rows, d_dy_,
                                  d_sq_, d_dz_[MLP_HIDDEN_LAYERS - 1], d_dzt_[MLP_HIDDEN_LAYERS - 1], stream_));
     SUR_TRY(grn_mlp_row_dot(d_ht_[MLP_HIDDEN_LAYERS - 1], d_dy_, MLP_WIDTH, rows, d_g_ + MLP_OFF_OUT_W, stream_));
     SUR_TRY(grn_mlp_sum(d_dy_, rows, d_g_ + MLP_OFF_OUT_B, stream_));
@@ -554,6 +599,12 @@ int grn_surrogate_train(void *sur, int steps, double *loss) {
   grn::Surrogate *s = as_sur(sur);
   if (!s) return grn::set_last_error(GRN_E_INVALID, "null surrogate");
   return s->train(steps, loss);
+}
+
+int grn_surrogate_set_pair_weight(void *sur, double weight) {
+  grn::Surrogate *s = as_sur(sur);
+  if (!s) return grn::set_last_error(GRN_E_INVALID, "null surrogate");
+  return s->set_pair_weight(weight);
 }
 
 int grn_surrogate_predict(void *sur, int n, const grn_model *models, double *log_errors) {

@@ -41,8 +41,18 @@ class Surrogate {
   int train_async(int steps);
   int finish();
   int predict(int n, const float *features, double *log_errors);
-  // The same on rows already encoded as BF16 (n x MLP_WIDTH).
+  // The same on rows already encoded as BF16 (n x MLP_WIDTH).  In pair mode
+  // each row is stored as a pair with itself (no pairwise information).
   int add_rows(int n, const uint16_t *rows, const double *log_errors);
+  // Pair mode (pair weight > 0): the store holds (child, parent) pairs and
+  // training adds the pairwise term of grn_mlp_out_backward, which fits the
+  // predicted change from parent to child to the simulated one.  The weight
+  // can change while the store is empty, or between two positive values.
+  int set_pair_weight(double w);
+  double pair_weight() const { return pair_w_; }
+  // n pairs: child rows and parent rows (n x MLP_WIDTH each) with their ln errors.
+  int add_pairs(int n, const uint16_t *child_rows, const uint16_t *parent_rows, const double *child_log_errors,
+                const double *parent_log_errors);
   int predict_rows(int n, const uint16_t *rows, double *log_errors);
   // Encodes models straight into BF16 rows (n x MLP_WIDTH), on host threads.
   void encode_rows(const std::vector<const Model *> &models, std::vector<uint16_t> *rows) const;
@@ -61,6 +71,7 @@ class Surrogate {
 
  private:
   int alloc();
+  int store_rows(int n, const uint16_t *rows, const float *targets);
   int upload_bf16_copies();
   int forward(int rows, bool keep_for_backward);
   int backward(int rows);
@@ -99,6 +110,7 @@ class Surrogate {
   uint64_t rng_ = 0;
   long adam_t_ = 0;
   float mu_ = 0.f, sigma_ = 1.f;
+  float pair_w_ = 0.f;
   grn_surrogate_stats stats_{};
 };
 

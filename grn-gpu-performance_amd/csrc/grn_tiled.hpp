@@ -123,22 +123,29 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
         for (int tx = 0; tx < ntx; ++tx) {
           const int ox = tx * SX - T, oy = ty * SY - T;
           const int gx = ox + f.x;
-          int gidx[CPT];
+          // cell k of the strip: global index, or -1 outside the grid
+          auto gidx = [&](int k) {
+            const int gy = oy + f.y0 + k;
+            return (f.active && gx >= 0 && gx < W && gy >= 0 && gy < H) ? gy * W + gx : -1;
+          };
+          f.nfb = 0u;
+          f.cellb = 0u;
 #pragma unroll
           for (int k = 0; k < CPT; ++k) {
             const int yt = f.y0 + k, gy = oy + yt;
-            const bool in = f.active && gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const bool in = gidx(k) >= 0;
             const bool interior = in && f.x >= T && f.x < T + SX && yt >= T && yt < T + SY;
-            f.nf[k] = in ? (float)((gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1)) : 0.f;
-            f.dtc[k] = in ? f.dt : 0.f;
-            f.trk[k] = interior ? 1.f : 0.f;
-            gidx[k] = in ? gy * W + gx : -1;
+            const unsigned nfk = in ? (unsigned)((gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1)) : 0u;
+            f.nfb |= nfk << (3 * k);
+            f.cellb |= (in ? 1u : 0u) << k;
+            f.cellb |= (interior ? 1u : 0u) << (16 + k);
           }
 #pragma unroll
           for (int k = 0; k < CPT; ++k) {
             // load from an in-range record, then select: no out-of-range
             // address is ever formed for cells outside the grid
-            const float4v *src = reinterpret_cast<const float4v *>(A + (size_t)(gidx[k] >= 0 ? gidx[k] : 0) * R);
+            const int gk = gidx(k);
+            const float4v *src = reinterpret_cast<const float4v *>(A + (size_t)(gk >= 0 ? gk : 0) * R);
             float r[R];
 #pragma unroll
             for (int j = 0; j < R / 4; ++j) {
@@ -150,8 +157,8 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-              f.hi[g][k] = gidx[k] >= 0 ? r[g] : 0.f;
-              f.lo[g][k] = gidx[k] >= 0 ? r[G + g] : 0.f;
+              f.hi[g][k] = gk >= 0 ? r[g] : 0.f;
+              f.lo[g][k] = gk >= 0 ? r[G + g] : 0.f;
             }
           }
           // (the previous tile's last step ended with a barrier after its
@@ -170,7 +177,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
           cblk = __builtin_fmaxf(cblk, c);
 #pragma unroll
           for (int k = 0; k < CPT; ++k)
-            if (f.trk[k] != 0.f) {
+            if (f.trkv(k)) {
               float r[R];
 #pragma unroll
               for (int g = 0; g < G; ++g) {
@@ -179,7 +186,7 @@ __device__ __forceinline__ void run_tiled_model(const GrnLaunch &a, const GrnPro
               }
 #pragma unroll
               for (int j = 2 * G; j < R; ++j) r[j] = 0.f;
-              float4v *dst = reinterpret_cast<float4v *>(B + (size_t)gidx[k] * R);
+              float4v *dst = reinterpret_cast<float4v *>(B + (size_t)gidx(k) * R);
 #pragma unroll
               for (int j = 0; j < R / 4; ++j) dst[j] = float4v{r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]};
             }

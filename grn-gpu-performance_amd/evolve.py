@@ -116,10 +116,12 @@ def coarse_dataset(ctx, prob: Problem, side: int):
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
            screen_policy: int = grn.SCREEN_ABSOLUTE, screen: str = "mlp", coarse_side: int = 8,
-           progress=None, results=None, name: str = "") -> Run:
+           pair_weight: float = 0.0, progress=None, results=None, name: str = "") -> Run:
     """Plain (screened False) or screened evolution.  screen "mlp": the
     learned surrogate scores the candidates; "coarse": a simulation of every
-    candidate on a coarse_side x coarse_side grid does.  results (optional):
+    candidate on a coarse_side x coarse_side grid does.  pair_weight > 0
+    trains the surrogate on (child, slot parent) pairs as well
+    (Surrogate.set_pair_weight).  results (optional):
     a store.Store that receives the run's parameters, every generation's
     statistics and the final Pareto front."""
     if screened and screen not in ("mlp", "coarse"):
@@ -137,6 +139,8 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
         if screen == "mlp":
             sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], batch=surrogate_batch,
                                 seed=prob.search["seed"])
+            if pair_weight > 0:
+                sur.set_pair_weight(pair_weight)
         search.set_oversample(oversample)
         search.set_screen_fraction(screen_fraction)
         search.set_screen_policy(screen_policy)

@@ -54,7 +54,7 @@ EXPORTED = [
     "grn_search_run_generation_screened", "grn_search_run_generation_coarse",
     "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
     "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
-    "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
+    "grn_surrogate_set_pair_weight", "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
     "grn_surrogate_get_params", "grn_surrogate_set_params", "grn_surrogate_get_stats",
     "grn_mlp_gemm_bf16",
 ]
@@ -134,7 +134,7 @@ class _ParetoEntry(ctypes.Structure):
 
 
 TUNE_ONCHIP_EXACT, TUNE_ONCHIP_ANY, TUNE_TILED = 0, 1, 2
-TUNE_MAX_CANDIDATES = 12
+TUNE_MAX_CANDIDATES = 24
 
 
 class _TuneCandidate(ctypes.Structure):
@@ -272,6 +272,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_surrogate_encode.argtypes = [ctypes.POINTER(_Model), ctypes.c_int, vp]
     lib.grn_surrogate_add.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.grn_surrogate_train.argtypes = [vp, ctypes.c_int, dp]
+    lib.grn_surrogate_set_pair_weight.argtypes = [vp, ctypes.c_double]
     lib.grn_surrogate_predict.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.grn_surrogate_predict_features.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.grn_surrogate_gradient.argtypes = [vp, ctypes.c_int, vp, vp, vp, dp]
@@ -856,6 +857,12 @@ class Surrogate:
         loss = ctypes.c_double()
         _check(self._lib.grn_surrogate_train(self._h, steps, ctypes.byref(loss)))
         return loss.value
+
+    def set_pair_weight(self, weight: float):
+        """Pair mode (weight > 0, set while the store is empty): children are
+        stored beside their slot parents and training also fits the predicted
+        change of ln error from parent to child."""
+        _check(self._lib.grn_surrogate_set_pair_weight(self._h, float(weight)))
 
     def predict(self, models) -> np.ndarray:
         out = np.zeros(max(len(models), 1), np.float64)

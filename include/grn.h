@@ -257,7 +257,7 @@ typedef enum {
   GRN_TUNE_TILED = 2         /* halo-tiled kernel */
 } grn_tune_kind;
 
-#define GRN_TUNE_MAX_CANDIDATES 12
+#define GRN_TUNE_MAX_CANDIDATES 24
 
 typedef struct {
   int kind;                 /* grn_tune_kind */
@@ -467,6 +467,13 @@ int grn_surrogate_encode(const grn_model *model, int n_morphogens, float *featur
 int grn_surrogate_add(void *surrogate, int n, const grn_model *models, const double *errors);
 /* Adam steps on minibatches drawn uniformly from the store. */
 int grn_surrogate_train(void *surrogate, int steps, double *last_loss);
+/* Pair mode (weight > 0; set while the store is empty): the store keeps each
+ * simulated child beside its slot parent, and training adds weight x the
+ * squared error of the predicted change of ln error from parent to child,
+ * the quantity a screen compares among one slot's candidates.  Samples from
+ * grn_surrogate_add are stored as pairs of a model with itself.  No
+ * counterpart in the reference. */
+int grn_surrogate_set_pair_weight(void *surrogate, double weight);
 /* Predicted ln error of each model. */
 int grn_surrogate_predict(void *surrogate, int n, const grn_model *models, double *log_errors);
 /* The same on encoded rows (n x 468 floats). */

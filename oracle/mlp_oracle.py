@@ -81,15 +81,24 @@ def forward(p, X, width: int, layers: int, bf16: bool = False):
     return y, (Z, H)
 
 
-def loss_and_grad(p, X, t, width: int, layers: int, bf16: bool = False):
-    """Mean squared error over the rows and its gradient (flat layout)."""
+def loss_and_grad(p, X, t, width: int, layers: int, bf16: bool = False, pair_w: float = 0.0):
+    """Mean squared error over the rows and its gradient (flat layout).
+    pair_w > 0: rows 2i and 2i + 1 are a pair and the loss adds
+    pair_w / n * sum over pairs of ((y_2i - y_2i+1) - (t_2i - t_2i+1))^2."""
     Wt, b, w, b0 = unpack(p, width, layers)
     q = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     y, (Z, H) = forward(p, X, width, layers, bf16)
     n = len(t)
-    e = y - np.asarray(t, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    e = y - t
     loss = float(np.mean(e * e))
     dy = 2.0 * e / n
+    if pair_w > 0:
+        assert n % 2 == 0
+        d = (y[0::2] - y[1::2]) - (t[0::2] - t[1::2])  # per pair, seen from its first row
+        loss += pair_w * float(np.sum(d * d)) / n
+        dy[0::2] += 2.0 * pair_w * d / n
+        dy[1::2] -= 2.0 * pair_w * d / n
     g = np.zeros(n_params(width, layers))
     mat = width * width
     ob, ow = layers * mat, layers * mat + layers * width

@@ -57,6 +57,9 @@ int Surrogate::train_async(int) { return GRN_E_INVALID; }
 int Surrogate::predict_rows(int, const uint16_t *, double *) { return GRN_E_INVALID; }
 uint16_t *Surrogate::staging_rows(size_t) { return nullptr; }
 int Surrogate::add_rows(int, const uint16_t *, const double *) { return GRN_E_INVALID; }
+int Surrogate::add_pairs(int, const uint16_t *, const uint16_t *, const double *, const double *) {
+  return GRN_E_INVALID;
+}
 void Surrogate::encode_rows(const std::vector<const Model *> &, uint16_t *) const {}
 }  // namespace grn
 

@@ -27,7 +27,7 @@ def mixed(ctx, seed=0, steps=120):
         target = workloads.gaussian_blob(side, side, side / 3, side / 2, side / 6)[None, None]
         ds = ctx.add_dataset(p, init, target)
         probs[ds] = (p, init, target)
-        ms = workloads._population(rng, 40, (3, 10), (2, 24), 3)
+        ms = workloads._population(rng, 40, (3, 12), (2, 24), 3)
         ids += [ds] * len(ms)
         models += ms
     return ids, models, probs

@@ -104,6 +104,35 @@ def test_gradient_matches_oracle(sur):
     assert g @ exact / (np.linalg.norm(g) * np.linalg.norm(exact)) > 0.99
 
 
+def test_pair_mode_gradient_matches_oracle():
+    """Pair mode: rows 2i, 2i + 1 are (child, parent) and the loss adds the
+    squared error of the predicted change; the device gradient against the
+    oracle's, with the same bars as the plain loss."""
+    rng = np.random.default_rng(12)
+    s = grn.Surrogate(0, n_morphogens=3, batch=1024, capacity=1 << 12, seed=8)
+    s.set_pair_weight(3.0)
+    p = rand_params(rng)
+    s.set_params(p)
+    F = rng.uniform(-1, 1, (200, grn.MLP_FEATURES)).astype(np.float32)
+    F[1::2] = F[0::2] + rng.normal(0, 0.05, F[1::2].shape).astype(np.float32)  # parents near their children
+    t = rng.normal(0, 1, 200)
+    loss, g = s.gradient(F, t)
+    want_loss, want = mo.loss_and_grad(p, padded(F), t, W, L, bf16=True, pair_w=3.0)
+    _, plain = mo.loss_and_grad(p, padded(F), t, W, L, bf16=True)
+    assert loss == pytest.approx(want_loss, rel=1e-3)
+    rel = np.linalg.norm(g - want) / np.linalg.norm(want)
+    assert rel < 1e-2, rel
+    assert np.linalg.norm(want - plain) > 10 * np.linalg.norm(g - want)  # the pair term is resolved
+    with pytest.raises(grn.GrnError):
+        s.gradient(F[:199], t[:199])  # pairs only
+    s.close()
+    s2 = grn.Surrogate(0, n_morphogens=3, batch=1024, capacity=1 << 12, seed=8)
+    s2.add([workloads.config0(seed=1, n_models=1).models[0]], [0.5])
+    with pytest.raises(grn.GrnError):
+        s2.set_pair_weight(1.0)  # the store already holds single rows
+    s2.close()
+
+
 def test_adam_step_matches_oracle():
     rng = np.random.default_rng(3)
     s = grn.Surrogate(0, n_morphogens=3, batch=128, capacity=16, seed=3, lr=1e-3)
@@ -334,5 +363,33 @@ def test_screened_generations_with_the_delta_policy():
     assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
     with pytest.raises(grn.GrnError):
         s.set_screen_policy(7)
+    sur.close()
+    ctx.close()
+
+
+def test_screened_generations_in_pair_mode():
+    """Pair mode in a screened search: the initial population is stored as
+    self-pairs, every later child beside its slot parent (two rows per
+    simulated child); the best error never rises and re-simulates to itself."""
+    ctx = grn.Context(0)
+    ds = search_problem(ctx, seed=10)
+    s = grn.Search(island_datasets=[ds, ds], n_islands=2, island_size=128, seed=11, **SEARCH)
+    sur = grn.Surrogate(0, n_morphogens=3, batch=512, capacity=1 << 14, seed=6)
+    sur.set_pair_weight(4.0)
+    s.set_oversample(4)
+    g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+    assert sc["n_simulated"] == 256 and sc["samples"] == 2 * 256
+    prev, samples = g["min_fit"], sc["samples"]
+    for gen in range(4):
+        print(f"[gpu] pair-mode screened generation {gen}", flush=True)
+        g, sc = s.run_generation_screened(ctx, sur, train_steps=20)
+        assert sc["n_simulated"] == 256
+        assert sc["samples"] == samples + 2 * 256
+        samples = sc["samples"]
+        assert np.isfinite(sc["loss"]) and -1.0 <= sc["spearman"] <= 1.0
+        assert g["min_fit"] <= prev
+        prev = g["min_fit"]
+    best, err = s.best()
+    assert ctx.evaluate_batch([best], datasets=ds)["fitness"][0] == err
     sur.close()
     ctx.close()

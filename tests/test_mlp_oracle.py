@@ -34,6 +34,32 @@ def test_gradient_matches_finite_differences():
     assert loss == pytest.approx(np.mean((mo.forward(p, X, width, layers)[0] - t) ** 2), rel=1e-15)
 
 
+def test_pair_term_matches_finite_differences_and_vanishes_for_equal_rows():
+    rng = np.random.default_rng(1)
+    width, layers, n, pw = 6, 3, 6, 2.5
+    p = rng.normal(0, 0.6, mo.n_params(width, layers))
+    X = rng.normal(0, 1, (n, width))
+    t = rng.normal(0, 1, n)
+    loss, g = mo.loss_and_grad(p, X, t, width, layers, pair_w=pw)
+    h = 1e-6
+    num = np.zeros_like(p)
+    for i in range(len(p)):
+        e = np.zeros_like(p)
+        e[i] = h
+        num[i] = (mo.loss_and_grad(p + e, X, t, width, layers, pair_w=pw)[0] -
+                  mo.loss_and_grad(p - e, X, t, width, layers, pair_w=pw)[0]) / (2 * h)
+    assert np.max(np.abs(num - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
+    # the pair term differs from the plain loss: it must change the gradient
+    assert np.max(np.abs(g - mo.loss_and_grad(p, X, t, width, layers)[1])) > 1e-3
+    # a pair of identical rows predicts no change: its term is the constant
+    # pw (t_a - t_b)^2 / n and leaves the gradient as it is without pairs
+    X2 = np.repeat(X[:n // 2], 2, axis=0)
+    l0, g0 = mo.loss_and_grad(p, X2, t, width, layers)
+    l1, g1 = mo.loss_and_grad(p, X2, t, width, layers, pair_w=pw)
+    assert np.max(np.abs(g1 - g0)) <= 1e-12 * np.max(np.abs(g0))
+    assert l1 - l0 == pytest.approx(pw * np.sum((t[0::2] - t[1::2]) ** 2) / n, rel=1e-12)
+
+
 def test_gelu_and_derivative():
     z = np.linspace(-6, 6, 1001)
     assert np.allclose(mo.gelu(z), z * norm.cdf(z), rtol=0, atol=1e-14)
