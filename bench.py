@@ -219,6 +219,11 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     evolve.make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
     search = grn.Search(island_datasets=[ds] * n_islands, **prob.search)
+    # tile and workgroup shapes per size class, timed on the initial population
+    models, dsi, _, _ = search.pending()
+    t0 = time.perf_counter()
+    reports = ctx.autotune(models, datasets=dsi, sample_models=256, step_cap=100)
+    tune_s = time.perf_counter() - t0
     sur = None
     if screened:
         sur = grn.Surrogate(0, n_morphogens=prob.search["n_morphogens"], seed=prob.search["seed"])
@@ -247,8 +252,9 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
         kernel_ms += st["kernel_ms"]
     wall = time.perf_counter() - t0
     ctx.set_launch_mode(grn.LAUNCH_EAGER)
+    ctx.clear_tuning()
     out = {"models_per_generation": n_islands * island_size, "screened": screened,
-           "generation0_s": gen0_s, "generation0_cell_steps_per_s": st0["cell_steps"] / gen0_s,
+           "autotune_s": tune_s, "autotuned_classes": len(reports), "generation0_s": gen0_s, "generation0_cell_steps_per_s": st0["cell_steps"] / gen0_s,
            "ms_per_generation": wall / steps * 1e3, "cell_steps_per_s": cell_steps / wall,
            "kernel_cell_steps_per_s": cell_steps / (kernel_ms / 1e3), "best_fitness": g["min_fit"]}
     if sur is not None:
