@@ -217,7 +217,7 @@ void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
                                               grn_field_table_w64, grn_field_table_wany, grn_field_table_wanyguard,
                                               grn_field_table_tiled512, grn_field_table_tiled,
                                               grn_field_table_tiled512s, grn_field_table_tiled512g,
-                                              grn_field_table_exact};
+                                              grn_field_table_tiled1024, grn_field_table_exact};
   for (auto fn : tables) {
     int n = 0;
     const GrnKernelEntry *t = fn(&n);

@@ -31,6 +31,7 @@ extern "C" const GrnKernelEntry *grn_field_table_tiled(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512s(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_tiled512g(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_tiled1024(int *n);
 // The FP64 kernel (grn_exact.hip): wc = -2.
 extern "C" const GrnKernelEntry *grn_field_table_exact(int *n);
 
