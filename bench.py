@@ -479,6 +479,9 @@ def main():
                     help="evolve: score candidates with the learned surrogate or a coarse-grid simulation")
     ap.add_argument("--coarse-side", type=int, default=8, help="evolve --screen coarse: side of the coarse grid")
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--path", choices=["auto", "lds", "tiled"], default="auto",
+                    help="config1: kernel path of the batch (tiled: force the halo-tiled kernels, for comparison)")
+    ap.add_argument("--sim-steps", type=int, default=0, help="config1: Euler steps (0: the workload's 2000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="config1: skip the config-2 generation and config-4 island reported beside the headline")
@@ -524,6 +527,9 @@ def main():
     import workloads
 
     w = workloads.config1(seed=args.seed, n_models=args.models)
+    if args.sim_steps > 0:
+        w.params.num_sim_steps = args.sim_steps
+    path = {"auto": grn.PATH_AUTO, "lds": grn.PATH_LDS, "tiled": grn.PATH_TILED}[args.path]
     ctx = grn.Context(0)
     workloads.make_target(ctx, w)
     ds = ctx.add_dataset(w.params, w.init, w.target)
@@ -541,7 +547,7 @@ def main():
         log(f"[bench] autotuned {len(reports)} size classes")
     # the programs are compiled and uploaded before the timed region: a step
     # launches the resident batch and collects its results
-    staged = ctx.stage_batch(batch, datasets=ds)
+    staged = ctx.stage_batch(batch, datasets=ds, path=path)
     log(f"[bench] warmup x{args.warmup}")
     for _ in range(args.warmup):
         staged.run()
@@ -563,7 +569,7 @@ def main():
     log(f"[bench] host-inclusive timing x{args.steps}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res_h = ctx.evaluate_batch(batch, datasets=ds)
+        res_h = ctx.evaluate_batch(batch, datasets=ds, path=path)
     host_inclusive = cell_steps / (time.perf_counter() - t0)
     assert np.array_equal(res_h, res)
     staged.close()
@@ -622,7 +628,8 @@ def main():
         "host_inclusive_cell_steps_per_s": host_inclusive,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": profiled_traffic() if args.models == 4096 else None,
+                     "traffic": profiled_traffic() if (args.models == 4096 and args.path == "auto"
+                                                        and args.sim_steps in (0, 2000)) else None,
                      "kernel_ms": float(np.mean(kernel_ms)),
                      "flops_per_cell_step_mean": float(np.mean(flops_per_model))},
         "autotune": tuning,
