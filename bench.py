@@ -344,8 +344,8 @@ def bench_time_to_fitness(args):
 
     seeds = [int(x) for x in args.seeds.split(",")] if args.seeds else [args.seed]
     ctx = grn.Context(0)
-    per_seed, ratios, sp, mae, phase = [], [], [], [], {"predict_ms": [], "simulate_ms": [], "breed_ms": [],
-                                                        "train_ms": []}
+    per_seed, ratios, sp, mae, phase = [], [], [], [], {"predict_ms": [], "simulate_ms": [], "record_ms": [],
+                                                        "breed_ms": [], "train_ms": []}
     gen_ms = {"plain": [], "screened": []}
     for seed in seeds:
         prob = evolve.scaled_config4(seed=seed)

@@ -88,6 +88,9 @@ struct Model {
   // bits (cell k at bit 3k); cellb bit k = the cell lies in the grid (time
   // step dt, else 0), bit 16 + k = its |R| counts towards the max change.
   unsigned nfb, cellb;
+  // TILED: the unpacked constants of the cells of the block being stepped
+  // (written at the start of each cell block, live only within it)
+  float blk_nf[TILED ? CPT : 1], blk_dt[TILED ? CPT : 1];
   float hi[G][CPT];
   float lo[G][CPT];
   Shared sh;
@@ -96,15 +99,30 @@ struct Model {
   __device__ __forceinline__ int pc() const { return WC ? WC + 2 : W + 2; }
   __device__ __forceinline__ float nfv(int k) const {
     if constexpr (TILED)
-      return (float)((nfb >> (3 * k)) & 7u);
+      return blk_nf[k];
     else
       return nf[k];
   }
   __device__ __forceinline__ float dtv(int k) const {
     if constexpr (TILED)
-      return ((cellb >> k) & 1u) ? dt : 0.f;
+      return blk_dt[k];
     else
       return dt;
+  }
+  // TILED: unpacks the constants of cells k0 .. k0 + n - 1 once per cell
+  // block, so the genes' updates read them instead of re-deriving them.
+  template <int n>
+  __device__ __forceinline__ void unpack_block(int k0) {
+    if constexpr (TILED) {
+      asm volatile("" : "+v"(nfb), "+v"(cellb));
+#pragma unroll
+      for (int j = 0; j < n; ++j) {
+        const int k = k0 + j;
+        blk_nf[k] = (float)((nfb >> (3 * k)) & 7u);
+        blk_dt[k] = ((cellb >> k) & 1u) ? dt : 0.f;
+        asm volatile("" : "+v"(blk_nf[k]), "+v"(blk_dt[k]));
+      }
+    }
   }
   __device__ __forceinline__ bool trkv(int k) const {
     if constexpr (TILED)
@@ -253,8 +271,8 @@ struct Model {
       // constants are not kept in scalar registers across blocks.
       const GrnProgram *Pq = P + opaque_zero();
       // (tiled: the packed per-cell constants are unpacked per block, not
-      // hoisted out of the step loop into one register per cell)
-      if constexpr (TILED) asm volatile("" : "+v"(nfb), "+v"(cellb));
+      // hoisted out of the step loop into registers for every cell)
+      unpack_block<CB>(k0);
       // log2 of every gene of this block's cells (their step-start values:
       // a cell's update touches only that cell)
       lg_t lg[CB];

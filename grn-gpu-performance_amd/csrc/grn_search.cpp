@@ -1159,6 +1159,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   if (rc) return rc;
   sc.simulate_ms = ms_since(t1);
   sc.n_simulated = n;
+  const auto t2 = clock::now();
   std::vector<double> raw(n);
   for (int i = 0; i < n; ++i) {
     h->errors[i] = h->results[i].fitness;
@@ -1240,6 +1241,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
   sc.train_ms = S->stats().train_ms;  // device time of the last completed training call
   sc.loss = S->stats().last_loss;     // and its loss
   sc.samples = S->stats().samples;
+  sc.record_ms = ms_since(t2);
   const auto t3 = clock::now();
   // the next candidates are encoded into the surrogate's pinned staging rows
   // by the breeding threads

@@ -184,7 +184,7 @@ class _ScreenStats(ctypes.Structure):
     _fields_ = [("n_candidates", ctypes.c_int), ("n_simulated", ctypes.c_int), ("spearman", ctypes.c_double),
                 ("mae", ctypes.c_double), ("loss", ctypes.c_double), ("predict_ms", ctypes.c_double),
                 ("train_ms", ctypes.c_double), ("simulate_ms", ctypes.c_double), ("breed_ms", ctypes.c_double),
-                ("samples", ctypes.c_long)]
+                ("samples", ctypes.c_long), ("record_ms", ctypes.c_double)]
 
 
 RESULT_DTYPE = np.dtype({"names": ["fitness", "raw_error", "change", "n_products", "steps",

@@ -502,6 +502,8 @@ typedef struct {
   double simulate_ms;       /* wall time of simulating the kept children */
   double breed_ms;          /* wall time of selection, migration and breeding */
   long samples;             /* samples in the surrogate's store */
+  double record_ms;         /* wall time of storing the simulated children as samples and
+                               enqueueing the training steps */
 } grn_screen_stats;
 
 /* Candidates per offspring slot (1 = the reference's search).  With k > 1
