@@ -264,6 +264,47 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     return out
 
 
+def measure_classes(ctx, seed, islands, island_size):
+    """The config-2 initial population split by (grid side, kernel the
+    autotuner chose): each group evaluated alone, as one launch, so its
+    kernel time, algorithmic TFLOP/s and cell-steps/s belong to that kernel
+    configuration.  Uses the context's current tuning."""
+    import grn
+    import workloads
+
+    w = workloads.config2(seed=seed, n_islands=islands, island_size=island_size)
+    ds = workloads.add_island_datasets(ctx, w)
+    search = grn.Search(island_datasets=ds, **w.search)
+    models, dsi, _, dts = search.pending()
+    reports = ctx.autotune(models, datasets=dsi, sample_models=256, step_cap=100)
+    choice = {}
+    for r in reports:
+        c = r["cand"][r["chosen"]]
+        choice[(r["dim_x"], r["genes"])] = (r["dim_x"], ["on-chip", "on-chip any width", "halo-tiled"][c["kind"]],
+                                            c["genes_lo"], c["genes_hi"], c["cells_per_thread"], c["block"],
+                                            c["tile_w"], c["tile_h"], c["halo"])
+    side_of = {d: w.sides[i] for i, d in enumerate(ds)}
+    genes = [m.info(w.search["n_morphogens"])[0] for m in models]
+    groups = {}
+    for i, m in enumerate(models):
+        groups.setdefault(choice[(side_of[int(dsi[i])], genes[i])], []).append(i)
+    out = []
+    for key in sorted(groups):
+        idx = groups[key]
+        sub = [models[i] for i in idx]
+        for _ in range(2):  # the second run is the one reported
+            ctx.evaluate_batch(sub, datasets=dsi[idx], dt=dts[idx])
+        st = ctx.stats()
+        tf = st["alg_flops"] / (st["kernel_ms"] / 1e3) / 1e12
+        out.append({"side": key[0], "kernel": key[1], "kernel_genes": [key[2], key[3]],
+                    "genes": sorted({genes[i] for i in idx}), "cells_per_thread": key[4], "block": key[5],
+                    "tile": [key[6], key[7]], "halo": key[8], "models": len(idx), "launches": st["n_launches"],
+                    "kernel_ms": st["kernel_ms"], "alg_tflops": tf, "frac_of_fp32_peak": tf / FP32_PEAK_TFLOPS,
+                    "kernel_cell_steps_per_s": st["cell_steps"] / (st["kernel_ms"] / 1e3)})
+    search.close()
+    return out
+
+
 def bench_islands(args):
     """The multi-island generation as the bench line's workload (one
     generation is one step)."""
@@ -298,6 +339,9 @@ def bench_islands(args):
         "best_fitness": r["best_fitness"],
         "cpu_baseline": None,
     }
+    if args.per_class:
+        log("[bench] per-class launches of the initial population")
+        out["classes"] = measure_classes(ctx, args.seed, args.islands, args.island_size)
     print(json.dumps(out))
     ctx.close()
 
@@ -500,6 +544,8 @@ def main():
     ap.add_argument("--pair-weight", type=float, default=4.0,
                     help="evolve: > 0 trains the surrogate on (child, slot parent) pairs as well")
     ap.add_argument("--no-tune", action="store_true", help="config2: skip the autotuner")
+    ap.add_argument("--per-class", action="store_true",
+                    help="config2: also time each (grid side, chosen kernel) group of the initial population alone")
     ap.add_argument("--tune", action="store_true", help="config1: autotune the size classes first")
     ap.add_argument("--tune-sample", type=int, default=256)
     ap.add_argument("--tune-steps", type=int, default=100)
