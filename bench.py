@@ -264,6 +264,17 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     return out
 
 
+def copy_probe(args):
+    """The plain copy kernel over --copy-probe-mib MiB (for calibrating the
+    HBM byte counters of a rocprofv3 --pmc run of the same command)."""
+    if args.copy_probe_mib:
+        import grn
+
+        ms = grn.copy_probe(args.copy_probe_mib << 20)
+        log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + written in {ms:.3f} ms "
+            f"({2 * (args.copy_probe_mib << 20) / ms / 1e6:.0f} GB/s)")
+
+
 def measure_classes(ctx, seed, islands, island_size):
     """The config-2 initial population split by (grid side, kernel the
     autotuner chose): each group evaluated alone, as one launch, so its
@@ -311,6 +322,7 @@ def bench_islands(args):
     import grn
 
     ctx = grn.Context(0)
+    copy_probe(args)
     r = measure_islands(ctx, args.seed, args.islands, args.island_size, args.warmup, args.steps,
                         tune=not args.no_tune, tune_sample=args.tune_sample, tune_steps=args.tune_steps)
     out = {
@@ -355,6 +367,7 @@ def bench_config4(args):
     import grn
 
     ctx = grn.Context(0)
+    copy_probe(args)
     plain = measure_config4(ctx, args.seed, args.islands4, args.island_size4, False, steps=args.steps)
     log(f"[bench] plain: {plain}")
     scr = measure_config4(ctx, args.seed, args.islands4, args.island_size4, True, steps=args.steps)
@@ -632,10 +645,7 @@ def main():
         plain_wall = time.perf_counter() - t0
         plain_rate = len(sample) * cells * w.params.num_sim_steps / plain_wall
 
-    if args.copy_probe_mib:
-        ms = grn.copy_probe(args.copy_probe_mib << 20)
-        log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + written in {ms:.3f} ms "
-            f"({2 * (args.copy_probe_mib << 20) / ms / 1e6:.0f} GB/s)")
+    copy_probe(args)
 
     cpu = None
     if not args.no_cpu:
