@@ -239,7 +239,7 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
     st0 = ctx.stats()
     for _ in range(warmup):
         search.run_generation_screened(ctx, sur, train_steps=20) if screened else search.run_generation(ctx)
-    cell_steps, kernel_ms = 0.0, 0.0
+    cell_steps, kernel_ms, alg_flops = 0.0, 0.0, 0.0
     t0 = time.perf_counter()
     for k in range(steps):
         log(f"[bench] config 4: timed generation {k + 1}")
@@ -250,13 +250,16 @@ def measure_config4(ctx, seed, n_islands, island_size, screened, warmup=0, steps
         st = ctx.stats()
         cell_steps += st["cell_steps"]
         kernel_ms += st["kernel_ms"]
+        alg_flops += st["alg_flops"]
     wall = time.perf_counter() - t0
     ctx.set_launch_mode(grn.LAUNCH_EAGER)
     ctx.clear_tuning()
     out = {"models_per_generation": n_islands * island_size, "screened": screened,
            "autotune_s": tune_s, "autotuned_classes": len(reports), "generation0_s": gen0_s, "generation0_cell_steps_per_s": st0["cell_steps"] / gen0_s,
            "ms_per_generation": wall / steps * 1e3, "cell_steps_per_s": cell_steps / wall,
-           "kernel_cell_steps_per_s": cell_steps / (kernel_ms / 1e3), "best_fitness": g["min_fit"]}
+           "kernel_cell_steps_per_s": cell_steps / (kernel_ms / 1e3),
+           "alg_tflops": alg_flops / (kernel_ms / 1e3) / 1e12,
+           "frac_of_fp32_peak": alg_flops / (kernel_ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS, "best_fitness": g["min_fit"]}
     if sur is not None:
         out["spearman"] = sc["spearman"]
         sur.close()
