@@ -533,7 +533,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--models", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--seeds", default="0,1,2,3,4,5,6,7", help="evolve: comma-separated problem/search seeds")
+    ap.add_argument("--seeds", default=",".join(str(i) for i in range(16)),
+                    help="evolve: comma-separated problem/search seeds")
     ap.add_argument("--screen-policy", type=int, default=0, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
     ap.add_argument("--screen", choices=["mlp", "coarse"], default="mlp",
                     help="evolve: score candidates with the learned surrogate or a coarse-grid simulation")
