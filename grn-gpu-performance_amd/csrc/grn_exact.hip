@@ -30,23 +30,23 @@
 
 namespace grn_exact {
 
-constexpr int kNT = 256;
-constexpr int kWaves = kNT / 64;
 constexpr double kMaxD = 1.7976931348623157e308;
 
+template <int NT>
 struct Reduce {
-  double wmax[2][kWaves];  // per-wave max |R|, by step parity
-  double wsum[kWaves];     // per-wave fitness partial sums
+  double wmax[2][NT / 64];  // per-wave max |R|, by step parity
+  double wsum[NT / 64];     // per-wave fitness partial sums
 };
 
 // MathAlgo::max (mathalgo.h:213-216): b if a < b, else a.
 __device__ __forceinline__ double ref_max(double a, double b) { return a < b ? b : a; }
 
-template <int G>
+template <int G, int kNT>
 __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *__restrict__ progs,
                                           const GrnDataset *__restrict__ datasets, int m, int pi,
-                                          double *lgs, Reduce &red) {
+                                          double *lgs, Reduce<kNT> &red) {
 #pragma clang fp contract(off)
+  constexpr int kWaves = kNT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GrnProgram *P = progs + pi;
   const GrnProgram64 *Q = a.progs64 + pi;
@@ -192,24 +192,24 @@ __device__ __forceinline__ void run_model(const GrnLaunch &a, const GrnProgram *
   }
 }
 
-template <int G, int GHI>
+template <int G, int GHI, int NT>
 __device__ __forceinline__ void dispatch(int g, const GrnLaunch &a, const GrnProgram *__restrict__ progs,
                                          const GrnDataset *__restrict__ datasets, int m, int pi, double *lgs,
-                                         Reduce &red) {
+                                         Reduce<NT> &red) {
   if constexpr (G <= GHI) {
     if (g == G)
-      run_model<G>(a, progs, datasets, m, pi, lgs, red);
+      run_model<G, NT>(a, progs, datasets, m, pi, lgs, red);
     else
-      dispatch<G + 1, GHI>(g, a, progs, datasets, m, pi, lgs, red);
+      dispatch<G + 1, GHI, NT>(g, a, progs, datasets, m, pi, lgs, red);
   }
 }
 
-template <int GLO, int GHI>
+template <int GLO, int GHI, int kNT>
 __global__ __launch_bounds__(kNT) void exact_kernel(GrnLaunch a, const GrnProgram *__restrict__ progs,
                                                     const GrnDataset *__restrict__ datasets,
                                                     const int *__restrict__ order) {
   __shared__ double s_lg[GHI * kNT];
-  __shared__ Reduce s_red;
+  __shared__ Reduce<kNT> s_red;
   __shared__ int s_model;
   GrnQueue *q = a.queue;
   const int n = __builtin_amdgcn_readfirstlane(q->n);
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kNT) void exact_kernel(GrnLaunch a, const GrnProgra
     if (m >= n) break;
     const int pi = __builtin_amdgcn_readfirstlane(ord[m]);
     const int g = __builtin_amdgcn_readfirstlane(progs[pi].G);
-    dispatch<GLO, GHI>(g, a, progs, datasets, m, pi, s_lg, s_red);
+    dispatch<GLO, GHI, kNT>(g, a, progs, datasets, m, pi, s_lg, s_red);
     __syncthreads();
   }
 }
@@ -230,8 +230,11 @@ __global__ __launch_bounds__(kNT) void exact_kernel(GrnLaunch a, const GrnProgra
 }  // namespace grn_exact
 
 // wc = -2 marks the FP64 kernel; it applies the diffusion guard always.
+// Two gene ranges: the 1-8-gene bodies need fewer registers (152 against
+// 204 VGPRs), so three workgroups share a CU instead of two.
 static const GrnKernelEntry kTable[] = {
-    {-2, 1, grn_exact::kNT, 1, 16, true, (const void *)&grn_exact::exact_kernel<1, 16>, 0},
+    {-2, 1, 256, 1, 8, true, (const void *)&grn_exact::exact_kernel<1, 8, 256>, 0},
+    {-2, 1, 256, 9, 16, true, (const void *)&grn_exact::exact_kernel<9, 16, 256>, 0},
 };
 
 extern "C" const GrnKernelEntry *grn_field_table_exact(int *n) {
