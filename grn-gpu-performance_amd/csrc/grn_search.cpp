@@ -1197,7 +1197,7 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int t
     for (int i = 0; i < n; ++i) {
       if (std::isnan(target[i])) continue;
       const grn::Individual *child = h->s->pending()[i];
-      const grn::Individual *par = cand_rows ? h->s->slot_parent(child) : nullptr;
+      const grn::Individual *par = h->s->slot_parent(child);  // null in the initial population
       const bool paired = par && !std::isnan(par->log_raw);
       idx.push_back(i);
       tc.push_back(target[i]);
