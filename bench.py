@@ -411,13 +411,15 @@ def bench_time_to_fitness(args):
         prob = evolve.scaled_config4(seed=seed)
         evolve.make_target(ctx, prob)
         runs = {}
-        for name, screened in (("plain", False), ("screened", True)):
+        for name, screened in (("plain", False), ("screened", args.screen != "none")):
             log(f"[bench] seed {seed}: {name} evolution, {args.generations} generations")
             runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-4,
                                        oversample=args.oversample, train_steps=args.train_steps,
                                        screen_fraction=args.screen_fraction, screen_policy=args.screen_policy,
-                                       screen=args.screen, coarse_side=args.coarse_side,
-                                       pair_weight=args.pair_weight if screened else 0.0, progress=log)
+                                       screen=args.screen if screened else "mlp", coarse_side=args.coarse_side,
+                                       pair_weight=args.pair_weight if screened else 0.0,
+                                       coarse_generations=args.coarse_generations if name == "screened" else 0,
+                                       progress=log)
         plain, scr = runs["plain"], runs["screened"]
         thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
         table = []
@@ -427,13 +429,17 @@ def bench_time_to_fitness(args):
                           "speedup": (tp / ts) if tp and ts else None})
         tp, ts = plain.time_to(plain.best[-1]), scr.time_to(plain.best[-1])
         ratios.append((tp / ts) if tp and ts else 0.0)
+        screened_gens = [x for x in scr.screen if "coarse_best" not in x]  # not the coarse-to-fine schedule's
+        coarse_gens = len(scr.screen) - len(screened_gens)
         if args.screen == "mlp":
-            sp += [x["spearman"] for x in scr.screen if x["n_candidates"]]
-            mae += [x["mae"] for x in scr.screen if x["n_candidates"]]
+            sp += [x["spearman"] for x in screened_gens if x["n_candidates"]]
+            mae += [x["mae"] for x in screened_gens if x["n_candidates"]]
             for k in phase:
-                phase[k] += [x[k] for x in scr.screen[2:]]
-        else:
-            phase.setdefault("coarse_ms", []).extend(x["coarse_ms"] for x in scr.screen[2:])
+                phase[k] += [x[k] for x in screened_gens[2:]]
+        elif args.screen == "coarse":
+            phase.setdefault("coarse_ms", []).extend(x["coarse_ms"] for x in screened_gens[2:])
+        if coarse_gens:
+            phase.setdefault("coarse_generation_ms", []).append(scr.wall[coarse_gens - 1] / coarse_gens * 1e3)
         gen_ms["plain"].append(plain.wall[-1] / len(plain.wall) * 1e3)
         gen_ms["screened"].append(scr.wall[-1] / len(scr.wall) * 1e3)
         per_seed.append({"seed": seed, "thresholds": table,
@@ -458,8 +464,9 @@ def bench_time_to_fitness(args):
         "config": {"workload": "time to fitness, config 4 at reduced size: " + evolve.scaled_config4().name,
                    "seeds": seeds, "oversample": args.oversample, "screen_fraction": args.screen_fraction,
                    "screen_policy": ["absolute", "delta", "better-half"][args.screen_policy],
-                   "screen": args.screen if args.screen == "mlp" else
+                   "screen": args.screen if args.screen != "coarse" else
                    f"coarse simulation on {args.coarse_side}x{args.coarse_side}",
+                   "coarse_generations": args.coarse_generations,
                    "train_steps_per_generation": args.train_steps if args.screen == "mlp" else None,
                    "pair_weight": args.pair_weight if args.screen == "mlp" else None},
         "speedup_per_seed": ratios,
@@ -536,9 +543,14 @@ def main():
     ap.add_argument("--seeds", default=",".join(str(i) for i in range(16)),
                     help="evolve: comma-separated problem/search seeds")
     ap.add_argument("--screen-policy", type=int, default=0, help="evolve: 0 absolute, 1 delta, 2 random among the better half")
-    ap.add_argument("--screen", choices=["mlp", "coarse"], default="mlp",
-                    help="evolve: score candidates with the learned surrogate or a coarse-grid simulation")
-    ap.add_argument("--coarse-side", type=int, default=8, help="evolve --screen coarse: side of the coarse grid")
+    ap.add_argument("--screen", choices=["mlp", "coarse", "none"], default="mlp",
+                    help="evolve: score candidates with the learned surrogate or a coarse-grid simulation "
+                         "(none: the second run is plain, with --coarse-generations)")
+    ap.add_argument("--coarse-side", type=int, default=8,
+                    help="evolve: side of the coarse grid (--screen coarse, --coarse-generations)")
+    ap.add_argument("--coarse-generations", type=int, default=0,
+                    help="evolve: the second run's first generations simulate on the coarse grid only "
+                         "(a coarse-to-fine schedule)")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--path", choices=["auto", "lds", "tiled"], default="auto",
                     help="config1: kernel path of the batch (tiled: force the halo-tiled kernels, for comparison)")

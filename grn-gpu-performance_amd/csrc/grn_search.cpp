@@ -636,6 +636,30 @@ const Individual *Search::slot_parent(const Individual *child) const {
   return nullptr;
 }
 
+int Search::island_of(const Individual *x) const {
+  for (int is = 0; is < p_.n_islands; ++is)
+    for (const std::vector<Individual> *v : {&islands_[is], &children_[is], &cand_[is]})
+      if (!v->empty() && x >= v->data() && x < v->data() + v->size()) return is;
+  return -1;
+}
+
+void Search::rescore() {
+  pending_.clear();
+  cand_ptrs_.clear();
+  for (std::vector<Individual> &isl : islands_)
+    for (Individual &ind : isl) {
+      ind.error = -1;
+      ind.parent_error = Individual().parent_error;  // no maxError cut, as for the initial population
+      ind.generation = -1;
+      ind.log_raw = ind.parent_log_raw = std::numeric_limits<double>::quiet_NaN();
+      pending_.push_back(&ind);
+    }
+  pareto_.clear();
+  best_perfect_ = -1;
+  last_generation_ = p_.n_generations > 0 ? p_.n_generations : -1;
+  rescoring_ = true;
+}
+
 void Search::choose_next_generation(int is, std::vector<Individual *> *winners) {
   // SearchAlgoDetCrowd::chooseNextGeneration (searchalgodetcrowd.cpp:103-147):
   // each child competes with its own parent and wins ties
@@ -762,10 +786,11 @@ void Search::advance(const double *errors) {
   // the Pareto front sees the initial population and every selected child
   // (Search::addNewIndividual, search.cpp:323-326)
   std::vector<Individual *> winners;
-  if (generation_ == 0)
+  if (generation_ == 0 || rescoring_)
     winners = pending_;
   else
     for (int is = 0; is < p_.n_islands; ++is) choose_next_generation(is, &winners);
+  rescoring_ = false;
   update_pareto(winners);
   // Early end once a perfect model is found (search.cpp:157-167): a new least
   // complex error-0 model at the head of the front moves the last generation
@@ -1071,36 +1096,54 @@ int grn_search_record_raw(void *search, const double *raw_errors) {
   return 0;
 }
 
-int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_datasets, double dif_scale,
-                                     int path, grn_generation_stats *stats, double *coarse_ms) {
-  SearchHandle *h = as_search(search);
-  if (!h || !ctx || !coarse_datasets) return grn::set_last_error(GRN_E_INVALID, "null argument");
+}  // extern "C"
+
+// Copies of the flattened models f (of the individuals list) with every
+// diffusion constant times dif_scale, each on island_datasets[its island],
+// in the handle's coarse arrays.
+static int scaled_copies(SearchHandle *h, const std::vector<grn::Individual *> &list, const FlatModels &f,
+                         const int *island_datasets, double dif_scale) {
+  const int n = (int)f.models.size();
+  std::vector<grn_gene> &genes = h->coarse_genes;
+  std::vector<grn_model> &models = h->coarse_models;
+  std::vector<int> &ds = h->coarse_ds;
+  genes.assign(f.genes.begin(), f.genes.end());
+  for (grn_gene &g : genes) g.dif *= dif_scale;
+  models.assign(f.models.begin(), f.models.end());
+  ds.resize(n);
+  for (int i = 0; i < n; ++i) {
+    models[i].genes = genes.data() + (f.models[i].genes - f.genes.data());
+    const int island = h->s->island_of(list[i]);
+    if (island < 0) return grn::set_last_error(GRN_E_INVALID, "model %d: no island", i);
+    ds[i] = island_datasets[island];
+  }
+  return 0;
+}
+
+static int check_scale(void *search, void *ctx, const int *datasets, double dif_scale) {
+  if (!search || !ctx || !datasets) return grn::set_last_error(GRN_E_INVALID, "null argument");
   if (!(dif_scale >= 0) || std::isinf(dif_scale))
     return grn::set_last_error(GRN_E_INVALID, "dif_scale must be finite and >= 0");
+  return 0;
+}
+
+extern "C" {
+
+int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_datasets, double dif_scale,
+                                     int path, grn_generation_stats *stats, double *coarse_ms) {
+  if (int rc = check_scale(search, ctx, coarse_datasets, dif_scale)) return rc;
+  SearchHandle *h = as_search(search);
   const auto t0 = std::chrono::steady_clock::now();
   if (h->pend.models.empty() && !h->s->candidates().empty()) {
-    const std::vector<grn::Individual *> &cl = h->s->candidates();
     const FlatModels &c = h->candidates();
     const int n = (int)c.models.size();
     // the candidates with every diffusion constant scaled to the coarse grid
-    std::vector<grn_gene> &genes = h->coarse_genes;
-    std::vector<grn_model> &models = h->coarse_models;
-    std::vector<int> &ds = h->coarse_ds;
-    genes.assign(c.genes.begin(), c.genes.end());
-    for (grn_gene &g : genes) g.dif *= dif_scale;
-    models.assign(c.models.begin(), c.models.end());
-    ds.resize(n);
-    const int n_islands = h->s->params().n_islands;
-    for (int i = 0; i < n; ++i) {
-      models[i].genes = genes.data() + (c.models[i].genes - c.genes.data());
-      const int island = (int)(cl[i]->id >> 40);
-      if (island < 0 || island >= n_islands) return grn::set_last_error(GRN_E_INVALID, "candidate %d: no island", i);
-      ds[i] = coarse_datasets[island];
-    }
+    if (int rc = scaled_copies(h, h->s->candidates(), c, coarse_datasets, dif_scale)) return rc;
     std::vector<grn_result> &r = h->coarse_res;
     r.resize(n);
     // each candidate keeps the time step of its full-size simulation
-    const int rc = grn_evaluate_batch(ctx, n, ds.data(), models.data(), nullptr, c.dts.data(), path, r.data());
+    const int rc = grn_evaluate_batch(ctx, n, h->coarse_ds.data(), h->coarse_models.data(), nullptr, c.dts.data(),
+                                      path, r.data());
     if (rc) return rc;
     std::vector<double> &sc = h->coarse_scores;
     sc.resize(n);
@@ -1113,6 +1156,33 @@ int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_
   }
   if (coarse_ms) *coarse_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return grn_search_run_generation(search, ctx, path, stats);
+}
+
+int grn_search_run_generation_scaled(void *search, void *ctx, const int *island_datasets, double dif_scale,
+                                     int path, grn_generation_stats *stats) {
+  if (int rc = check_scale(search, ctx, island_datasets, dif_scale)) return rc;
+  SearchHandle *h = as_search(search);
+  if (h->pend.models.empty() && !h->s->candidates().empty())
+    return grn::set_last_error(GRN_E_INVALID, "candidates await grn_search_screen");
+  const int n = (int)h->pend.models.size();
+  if (n == 0) return grn_search_advance(search, nullptr, stats);
+  if (int rc = scaled_copies(h, h->s->pending(), h->pend, island_datasets, dif_scale)) return rc;
+  h->results.resize(n);
+  // each model keeps its own time step and its slot parent's error as maxError
+  const int rc = grn::evaluate_batch_hooked(ctx, n, h->coarse_ds.data(), h->coarse_models.data(),
+                                            h->pend.max_errors.data(), h->pend.dts.data(), path, h->results.data(),
+                                            generation_stage, h);
+  if (rc) return rc;
+  if (stats) *stats = h->s->stats();
+  return 0;
+}
+
+int grn_search_rescore(void *search) {
+  SearchHandle *h = as_search(search);
+  if (!h) return grn::set_last_error(GRN_E_INVALID, "null search");
+  h->s->rescore();
+  h->flatten();
+  return 0;
 }
 
 int grn_search_run_generation_screened(void *search, void *ctx, void *sur, int train_steps, int path,

@@ -117,6 +117,15 @@ class Search {
   // The parent a pending child competes with (its slot's parent), or null
   // for a member of the initial population.  Valid until advance().
   const Individual *slot_parent(const Individual *child) const;
+  // The island whose population, children or candidates hold x (-1: none).
+  int island_of(const Individual *x) const;
+  // Puts every island member back into pending() the way an initial
+  // population is: the children or candidates bred for the next generation
+  // are dropped, the Pareto front and the perfect-model end are cleared, and
+  // the next advance() records the members' errors without selection.  For a
+  // change of fidelity: members scored on a coarse grid scored again on the
+  // full one.
+  void rescore();
 
   const grn_search_params &params() const { return p_; }
   int generation() const { return generation_; }
@@ -196,6 +205,7 @@ class Search {
   // Search::calcParalEvolution's end of the run (search.cpp:128-167)
   int last_generation_ = -1;   // -1: no limit
   int best_perfect_ = -1;      // complexity of the least complex error-0 model on the front
+  bool rescoring_ = false;     // pending() holds the members (rescore())
 };
 
 }  // namespace grn

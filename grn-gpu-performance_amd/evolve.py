@@ -10,6 +10,7 @@ max_generations.  Every error the search keeps comes from the simulator.
 """
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -116,20 +117,28 @@ def coarse_dataset(ctx, prob: Problem, side: int):
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
            screen_policy: int = grn.SCREEN_ABSOLUTE, screen: str = "mlp", coarse_side: int = 8,
-           pair_weight: float = 0.0, progress=None, results=None, name: str = "") -> Run:
+           pair_weight: float = 0.0, coarse_generations: int = 0, progress=None, results=None,
+           name: str = "") -> Run:
     """Plain (screened False) or screened evolution.  screen "mlp": the
     learned surrogate scores the candidates; "coarse": a simulation of every
     candidate on a coarse_side x coarse_side grid does.  pair_weight > 0
     trains the surrogate on (child, slot parent) pairs as well
-    (Surrogate.set_pair_weight).  results (optional):
+    (Surrogate.set_pair_weight).  coarse_generations > 0: a coarse-to-fine
+    schedule -- the first coarse_generations generations simulate every
+    child on the coarse_side grid only (Search.run_generation_scaled; their
+    best counts as no simulator-verified error, so run.best holds inf), then
+    the population is scored at full size (Search.rescore) and the run goes
+    on as configured.  results (optional):
     a store.Store that receives the run's parameters, every generation's
     statistics and the final Pareto front."""
     if screened and screen not in ("mlp", "coarse"):
         raise ValueError(f"unknown screen {screen!r}")
+    if coarse_generations < 0 or coarse_generations >= max_generations:
+        raise ValueError("coarse_generations must lie in [0, max_generations)")
     if prob.target is None:
         make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
-    if screened and screen == "coarse":
+    if (screened and screen == "coarse") or coarse_generations:
         ds_c, dif_scale = coarse_dataset(ctx, prob, coarse_side)
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     n_isl = prob.search["n_islands"]
@@ -141,7 +150,7 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
                                 seed=prob.search["seed"])
             if pair_weight > 0:
                 sur.set_pair_weight(pair_weight)
-        search.set_oversample(oversample)
+        search.set_oversample(1 if coarse_generations else oversample)
         search.set_screen_fraction(screen_fraction)
         search.set_screen_policy(screen_policy)
     run = Run(screened)
@@ -150,6 +159,17 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
                                    results.add_search_params(search.params), n_isl)
     t0 = time.perf_counter()
     for gen in range(max_generations):
+        if gen < coarse_generations:
+            g = search.run_generation_scaled(ctx, [ds_c] * n_isl, dif_scale)
+            run.cell_steps += ctx.stats()["cell_steps"]
+            run.best.append(math.inf)
+            run.wall.append(time.perf_counter() - t0)
+            run.screen.append({"coarse_best": g["min_fit"]})
+            if gen + 1 == coarse_generations:  # to full size: breed k candidates from the re-scored members
+                if screened:
+                    search.set_oversample(oversample)
+                search.rescore()
+            continue
         if screened and screen == "coarse":
             g, cms = search.run_generation_coarse(ctx, [ds_c] * n_isl, dif_scale)
             run.screen.append({"coarse_ms": cms})

@@ -52,6 +52,7 @@ EXPORTED = [
     "grn_search_set_oversample", "grn_search_candidates", "grn_search_screen",
     "grn_search_set_screen_fraction", "grn_search_set_screen_policy", "grn_search_record_raw",
     "grn_search_run_generation_screened", "grn_search_run_generation_coarse",
+    "grn_search_run_generation_scaled", "grn_search_rescore",
     "grn_surrogate_default_params", "grn_surrogate_create", "grn_surrogate_destroy",
     "grn_surrogate_n_params", "grn_surrogate_encode", "grn_surrogate_add", "grn_surrogate_train",
     "grn_surrogate_set_pair_weight", "grn_surrogate_predict", "grn_surrogate_predict_features", "grn_surrogate_gradient",
@@ -262,6 +263,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.grn_search_run_generation_coarse.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int,
                                                      ctypes.POINTER(_GenerationStats),
                                                      ctypes.POINTER(ctypes.c_double)]
+    lib.grn_search_run_generation_scaled.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int,
+                                                     ctypes.POINTER(_GenerationStats)]
+    lib.grn_search_rescore.argtypes = [vp]
     lib.grn_surrogate_default_params.argtypes = [ctypes.POINTER(_SurrogateParams)]
     lib.grn_surrogate_default_params.restype = None
     lib.grn_surrogate_create.argtypes = [ctypes.c_int, ctypes.POINTER(_SurrogateParams), ctypes.POINTER(vp)]
@@ -753,6 +757,26 @@ class Search:
         _check(self._lib.grn_search_run_generation_coarse(self._h, ctx._h, ds.ctypes.data, float(dif_scale), path,
                                                           ctypes.byref(st), ctypes.byref(ms)))
         return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}, ms.value
+
+    def run_generation_scaled(self, ctx: "Context", island_datasets, dif_scale: float,
+                              path: int = PATH_AUTO) -> dict:
+        """One plain generation whose pending models are simulated on
+        island_datasets[island] (say a coarser grid of the island's
+        experiments) with their diffusion constants times dif_scale; see
+        grn_search_run_generation_scaled."""
+        ds = np.ascontiguousarray(island_datasets, dtype=np.int32)
+        if ds.shape != (self.params["n_islands"],):
+            raise ValueError(f"expected {self.params['n_islands']} dataset ids")
+        st = _GenerationStats()
+        _check(self._lib.grn_search_run_generation_scaled(self._h, ctx._h, ds.ctypes.data, float(dif_scale), path,
+                                                          ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _GenerationStats._fields_}
+
+    def rescore(self):
+        """Every island member pending again, as an initial population (the
+        next generation records their errors without selection): after a
+        change of fidelity."""
+        _check(self._lib.grn_search_rescore(self._h))
 
     def pareto(self) -> list:
         """The Pareto front on (error, complexity in use) as

@@ -556,6 +556,21 @@ int grn_search_run_generation_screened(void *search, void *ctx, void *surrogate,
  * coarse pass and the screen. */
 int grn_search_run_generation_coarse(void *search, void *ctx, const int *coarse_datasets, double dif_scale,
                                      int path, grn_generation_stats *stats, double *coarse_ms);
+/* One plain generation simulated at another fidelity (no counterpart in the
+ * reference, whose search simulates every model at one size): the pending
+ * models run on island_datasets[island] with each diffusion constant
+ * multiplied by dif_scale, their own time steps and maxErrors, and the
+ * search advances on those errors.  A search run this way for its first
+ * generations on a coarse grid, then grn_search_rescore'd, continues at full
+ * size from the coarse population (a coarse-to-fine schedule). */
+int grn_search_run_generation_scaled(void *search, void *ctx, const int *island_datasets, double dif_scale,
+                                     int path, grn_generation_stats *stats);
+/* Makes every island member pending again, as an initial population is:
+ * the children or candidates bred for the next generation are dropped, the
+ * Pareto front and the perfect-model end are cleared, and the next
+ * generation records the members' errors without selection, then breeds.
+ * Used after a change of fidelity (grn_search_run_generation_scaled). */
+int grn_search_rescore(void *search);
 
 #ifdef __cplusplus
 }

@@ -113,3 +113,65 @@ def test_coarse_screened_evolution_is_verified_and_deterministic():
     with pytest.raises(ValueError):
         evolve.evolve(ctx, prob, True, max_generations=1, threshold=0.0, screen="coarse", coarse_side=12)
     ctx.close()
+
+
+def test_coarse_to_fine_generations_equal_their_composition():
+    """grn_search_run_generation_scaled advances on the errors the public
+    calls give (pending models with scaled diffusion on the coarse dataset,
+    their own time steps and maxErrors), and after grn_search_rescore the
+    members are simulated at full size as an initial population is."""
+    print("\n[gpu] coarse-to-fine generations", flush=True)
+    ctx = grn.Context(0)
+    prob = evolve.scaled_config4(seed=3, n_islands=2, island_size=64, steps=300)
+    evolve.make_target(ctx, prob)
+    ds = ctx.add_dataset(prob.params, prob.init, prob.target)
+    ds_c, scale = evolve.coarse_dataset(ctx, prob, 8)
+    a = grn.Search(island_datasets=[ds, ds], **prob.search)
+    b = grn.Search(island_datasets=[ds, ds], **prob.search)
+    for gen in range(4):
+        fine = gen == 3
+        if gen == 3:
+            a.rescore()
+            b.rescore()
+        if fine:
+            ga = a.run_generation(ctx)
+        else:
+            ga = a.run_generation_scaled(ctx, [ds_c, ds_c], scale)
+        models, dss, me, dts = b.pending()
+        if fine:
+            assert np.all(me == 1000.0) and len(models) == 128
+            errs = ctx.evaluate_batch(models, datasets=dss, max_errors=me, dt=dts)["fitness"]
+        else:
+            cm = []
+            for m in models:
+                g = m.genes.copy()
+                g["dif"] = g["dif"] * scale
+                cm.append(grn.Model(g, m.links))
+            errs = ctx.evaluate_batch(cm, datasets=[ds_c] * len(cm), max_errors=me, dt=dts)["fitness"]
+        gb = b.advance(errs)
+        assert ga == gb, gen
+        pa, pb = a.pending(), b.pending()
+        assert [m.to_text(17) for m in pa[0]] == [m.to_text(17) for m in pb[0]], gen
+        assert np.array_equal(pa[2], pb[2])
+    a.close()
+    b.close()
+    ctx.close()
+
+
+def test_coarse_to_fine_evolution_is_verified_and_deterministic():
+    print("\n[gpu] coarse-to-fine evolution", flush=True)
+    ctx = grn.Context(0)
+    prob = evolve.scaled_config4(seed=2, n_islands=4, island_size=128, steps=400)
+    evolve.make_target(ctx, prob)
+    for screened, kw in ((False, {}), (True, dict(oversample=4, screen="coarse"))):
+        runs = [evolve.evolve(ctx, prob, screened, max_generations=14, threshold=0.0, coarse_side=8,
+                              coarse_generations=6, **kw) for _ in range(2)]
+        a, b = runs
+        assert a.best == b.best and a.verified and b.verified
+        assert a.best[:6] == [float("inf")] * 6 and np.isfinite(a.best[6])
+        assert all(x >= y for x, y in zip(a.best[6:], a.best[7:]))
+        assert [x["coarse_best"] for x in a.screen[:6]] == [x["coarse_best"] for x in b.screen[:6]]
+        check_best_with_oracle(prob, a)
+    with pytest.raises(ValueError):
+        evolve.evolve(ctx, prob, False, max_generations=4, threshold=0.0, coarse_generations=4)
+    ctx.close()

@@ -335,6 +335,44 @@ def test_coarse_generation_rejects_bad_arguments():
                                                     None, None) == -1
     with pytest.raises(ValueError):
         s.run_generation_coarse(None, [0, 0, 0], 0.25)  # one coarse dataset per island
+    # the same checks guard a generation simulated at another fidelity
+    assert lib.grn_search_run_generation_scaled(s._h, None, ds.ctypes.data, 0.25, grn.PATH_AUTO, None) == -1
+    assert lib.grn_search_run_generation_scaled(s._h, fake_ctx, None, 0.25, grn.PATH_AUTO, None) == -1
+    for bad in (-1.0, float("inf"), float("nan")):
+        assert lib.grn_search_run_generation_scaled(s._h, fake_ctx, ds.ctypes.data, bad, grn.PATH_AUTO,
+                                                    None) == -1
+    with pytest.raises(ValueError):
+        s.run_generation_scaled(None, [0], 0.25)
+    assert lib.grn_search_rescore(None) == -1
+    s.close()
+
+
+def test_rescore_scores_the_members_again():
+    """grn_search_rescore makes the members pending as an initial population
+    is: the next advance records their new errors without selection (the
+    statistics and the front see only those), with no maxError cut, then
+    breeding goes on from them."""
+    s = grn.Search(n_islands=2, island_size=8, n_morphogens=3, seed=7, migration=grn.MIGRATION_NONE)
+    hist = drive(s, 3)
+    before = hist[-1][0]
+    s.rescore()
+    models, ds, me, dt = s.pending()
+    assert len(models) == 16 and np.all(me == 1000.0)
+    errs = np.array([fake_error(m) for m in models])
+    # the pending models are the population: the same errors give the same statistics
+    assert errs.min() == before["min_fit"] and errs.max() == before["max_fit"]
+    assert abs(errs.mean() - before["mean_fit"]) <= 1e-12
+    # scored again (say at another fidelity): every member survives with its new error
+    new = errs + 5.0
+    st = s.advance(new)
+    assert st["generation"] == before["generation"] + 1 and st["n_evaluated"] == 16
+    assert st["min_fit"] == new.min() and st["max_fit"] == new.max()
+    assert all(e["error"] in set(new) for e in s.pareto())
+    # the next children compete with the re-scored parents
+    kids, _, me2, _ = s.pending()
+    assert len(kids) == 16 and set(me2) <= set(new)
+    st2 = s.advance(np.array([fake_error(m) + 5.0 for m in kids]))
+    assert st2["max_fit"] <= new.max()  # crowding keeps the better of child and parent
     s.close()
 
 
