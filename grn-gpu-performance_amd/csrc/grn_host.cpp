@@ -29,6 +29,7 @@
 #include "grn_internal.hpp"
 #include "grn_kernels.h"
 #include "grn_model.hpp"
+#include "grn_pool.hpp"
 #include "grn_program.h"
 
 // The batch buffers of grn_evaluate_batch_noalloc: one fixed array per device
@@ -213,7 +214,7 @@ struct Ctx {
 
 void collect_kernels(std::vector<const GrnKernelEntry *> *out) {
   const GrnKernelEntry *(*tables[])(int *) = {grn_field_table_w32, grn_field_table_w32wide, grn_field_table_w32wide9,
-                                              grn_field_table_w32wide13, grn_field_table_w48,
+                                              grn_field_table_w32wide13, grn_field_table_w8, grn_field_table_w48,
                                               grn_field_table_w64, grn_field_table_wany, grn_field_table_wanyguard,
                                               grn_field_table_tiled512, grn_field_table_tiled,
                                               grn_field_table_tiled512s, grn_field_table_tiled512g,
@@ -402,6 +403,7 @@ struct Plan {
   std::vector<Item> items;  // input order
   std::vector<GrnProgram64> p64;  // [n] when a model runs on the FP64 kernel (else empty)
   std::vector<int> idx;
+  std::vector<int> rank;  // per item: its kernel class's rank (the order of the launches)
   std::vector<std::pair<int, int>> groups;  // [begin, end) in idx order
   struct Launch {
     KernelChoice kc;
@@ -462,14 +464,9 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
       if (!crc[i]) crc[i] = grn::validate_program(items[i].prog, (int)c->datasets.size(), &cerr[i]);
     }
   };
-  const int n_threads = n >= 1024 ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
-  if (n_threads > 1) {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < n_threads; ++t) pool.emplace_back(compile_range, t, n_threads);
-    for (std::thread &th : pool) th.join();
-  } else {
-    compile_range(0, 1);
-  }
+  // (the host pool's threads for large batches, in strided chunks)
+  const int n_chunks = n >= 256 ? grn::host_threads() : 1;
+  grn::parallel_for(n_chunks, [&](int t) { compile_range(t, n_chunks); });
   for (int i = 0; i < n; ++i) {
     if (crc[i]) return fail(crc[i], "model %d: %s", i, cerr[i].c_str());
     const int ds = items[i].dataset;
@@ -509,9 +506,21 @@ int make_plan(Ctx *c, int n, const int *dataset_ids, const grn_model *models, co
     return std::make_tuple((const void *)x.k, x.tile.tw, x.tile.th, x.tile.sx, x.tile.sy, x.tile.halo,
                            x.k->wc == -1 ? pa.dim_x : 0, x.k->wc == -1 ? pa.dim_y : 0);
   };
+  // each item's class as its rank among the batch's (few) distinct keys, so
+  // the sort compares two ints, not two tuples built per comparison
+  std::vector<decltype(class_key(0))> keys;
+  for (int i = 0; i < n; ++i)
+    if (i == 0 || class_key(i) != class_key(i - 1)) keys.push_back(class_key(i));
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  std::vector<int> &rank = P->rank;
+  rank.resize(n);
+  for (int i = 0; i < n; ++i)
+    rank[i] = (i > 0 && class_key(i) == class_key(i - 1))
+                  ? rank[i - 1]
+                  : (int)(std::lower_bound(keys.begin(), keys.end(), class_key(i)) - keys.begin());
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-    const auto ka = class_key(a), kb = class_key(b);
-    if (ka != kb) return ka < kb;
+    if (rank[a] != rank[b]) return rank[a] < rank[b];
     return items[a].cost > items[b].cost;
   });
 
@@ -586,10 +595,17 @@ int upload_plan(Ctx *c, const Plan &P, BatchBufs &B, std::vector<double> *maxerr
     c->h_progs_cap = cap;
   }
   maxerr->resize(n);
-  for (int i = 0; i < n; ++i) {
-    c->h_progs[i] = P.items[i].prog;
-    (*maxerr)[i] = P.items[i].max_err;
-  }
+  // 2 KB per program: a large batch is copied to the staging buffer by
+  // several threads (the copy is bound by host memory bandwidth)
+  auto stage_range = [&](int t, int nt) {
+    const int b = (int)((long)n * t / nt), e = (int)((long)n * (t + 1) / nt);
+    for (int i = b; i < e; ++i) {
+      c->h_progs[i] = P.items[i].prog;
+      (*maxerr)[i] = P.items[i].max_err;
+    }
+  };
+  const int n_copy = n >= 2048 ? 8 : 1;
+  grn::parallel_for(n_copy, [&](int t) { stage_range(t, n_copy); });
   if (n > 0) {
     HIP_TRY(hipMemcpyAsync(B.progs.p, c->h_progs, sizeof(GrnProgram) * n, hipMemcpyHostToDevice, st));
     if (!P.p64.empty())

@@ -23,6 +23,7 @@ extern "C" const GrnKernelEntry *grn_field_table_w32(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w32wide(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w32wide9(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w32wide13(int *n);
+extern "C" const GrnKernelEntry *grn_field_table_w8(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w48(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_w64(int *n);
 extern "C" const GrnKernelEntry *grn_field_table_wany(int *n);

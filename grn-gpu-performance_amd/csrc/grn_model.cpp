@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -14,9 +15,51 @@
 namespace grn {
 
 std::vector<int> Model::labels_in_use(int n_morphogens) const {
-  // Breadth-first from the structural labels 0..nMorphogens-1 over
-  // regulated -> regulator edges (model.cpp:243-270).
-  // (models hold tens of labels: a linear membership test beats a tree)
+  // The labels reachable from the structural labels 0..nMorphogens-1 over
+  // regulated -> regulator edges (model.cpp:243-270).  A model names tens of
+  // labels: numbered in a table of at most 64, the closure is a fixpoint on
+  // a 64-bit set over the links' (regulated, regulator) pairs; a model with
+  // more distinct labels takes the breadth-first walk below.
+  constexpr int kTab = 64, kEdges = 256;
+  if (n_morphogens >= 0 && n_morphogens <= kTab && links.size() <= (size_t)kEdges) {
+    int tab[kTab];
+    int nt = n_morphogens;
+    for (int i = 0; i < nt; ++i) tab[i] = i;
+    auto id = [&](int lab) {
+      for (int i = 0; i < nt; ++i)
+        if (tab[i] == lab) return i;
+      if (nt == kTab) return -1;
+      tab[nt] = lab;
+      return nt++;
+    };
+    uint8_t from[kEdges], to[kEdges];
+    bool ok = true;
+    const int ne = (int)links.size();
+    for (int e = 0; e < ne && ok; ++e) {
+      const int a = id(links[e].regulated), b = id(links[e].regulator);
+      ok = a >= 0 && b >= 0;
+      from[e] = (uint8_t)a;
+      to[e] = (uint8_t)b;
+    }
+    if (ok) {
+      uint64_t set = n_morphogens == 64 ? ~0ull : (1ull << n_morphogens) - 1;
+      for (bool grew = true; grew;) {
+        grew = false;
+        for (int e = 0; e < ne; ++e)
+          if (((set >> from[e]) & 1) && !((set >> to[e]) & 1)) {
+            set |= 1ull << to[e];
+            grew = true;
+          }
+      }
+      std::vector<int> out;
+      out.reserve(__builtin_popcountll(set));
+      for (int i = 0; i < nt; ++i)
+        if ((set >> i) & 1) out.push_back(tab[i]);
+      std::sort(out.begin(), out.end());
+      return out;
+    }
+  }
+  // (a linear membership test beats a tree at these sizes)
   std::vector<int> order;
   order.reserve(n_morphogens + links.size());
   for (int i = 0; i < n_morphogens; ++i) order.push_back(i);

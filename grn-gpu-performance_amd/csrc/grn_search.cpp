@@ -17,6 +17,7 @@
 #include <thread>
 
 #include "grn_internal.hpp"
+#include "grn_pool.hpp"
 #include "grn_surrogate.hpp"
 
 namespace grn {
@@ -350,33 +351,6 @@ static uint64_t mix64(uint64_t x) {
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
   return x ^ (x >> 31);
-}
-
-// Host threads for breeding: the CPUs this process may run on (its affinity
-// mask), at most 16 -- the host share that goes with one GPU.
-static int host_threads() {
-  cpu_set_t set;
-  int n = (int)std::thread::hardware_concurrency();
-  if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
-  return std::max(1, std::min(16, n));
-}
-
-// f(0) ... f(n - 1) on up to host_threads() threads, dynamically scheduled.
-template <class F>
-static void parallel_for(int n, F f) {
-  const int n_threads = std::min(n, host_threads());
-  if (n_threads <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int> next{0};
-  auto work = [&] {
-    for (int i; (i = next.fetch_add(1)) < n;) f(i);
-  };
-  std::vector<std::thread> pool;
-  for (int t = 1; t < n_threads; ++t) pool.emplace_back(work);
-  work();
-  for (std::thread &th : pool) th.join();
 }
 
 Search::Search(const grn_search_params &p, const int *island_datasets) : p_(p) {

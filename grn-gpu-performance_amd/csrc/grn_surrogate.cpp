@@ -10,6 +10,7 @@
 #include <memory>
 #include <thread>
 
+#include "grn_pool.hpp"
 #include "grn_search.hpp"
 
 namespace grn {
@@ -312,19 +313,11 @@ void Surrogate::encode_rows(const std::vector<const Model *> &models, std::vecto
 
 void Surrogate::encode_rows(const std::vector<const Model *> &models, uint16_t *rows) const {
   const size_t n = models.size();
-  // at most 16 threads: the host share that goes with one GPU
-  const size_t hw = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
-  const int n_threads = (int)std::max<size_t>(1, std::min<size_t>(hw, (n + 255) / 256));
-  auto work = [&](int t) {
-    for (size_t i = t; i < n; i += n_threads) encode_model_bf16(*models[i], p_.n_morphogens, rows + i * MLP_WIDTH);
-  };
-  if (n_threads == 1) {
-    work(0);
-    return;
-  }
-  std::vector<std::thread> pool;
-  for (int t = 0; t < n_threads; ++t) pool.emplace_back(work, t);
-  for (std::thread &th : pool) th.join();
+  // (the host pool's threads, at least 256 rows per chunk)
+  const int n_chunks = (int)std::max<size_t>(1, std::min<size_t>(host_threads(), (n + 255) / 256));
+  parallel_for(n_chunks, [&](int t) {
+    for (size_t i = t; i < n; i += n_chunks) encode_model_bf16(*models[i], p_.n_morphogens, rows + i * MLP_WIDTH);
+  });
 }
 
 uint16_t *Surrogate::staging_rows(size_t n) {

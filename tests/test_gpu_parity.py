@@ -108,7 +108,8 @@ def test_plain_equals_batched_bitwise(ctx):
 
 
 @pytest.mark.parametrize("W,H,gr", [(32, 32, (3, 16)), (48, 48, (3, 12)), (64, 64, (4, 9)),
-                                    (16, 16, (3, 8)), (20, 12, (3, 8)), (7, 5, (3, 16)), (8, 8, (3, 16))])
+                                    (16, 16, (3, 8)), (20, 12, (3, 8)), (7, 5, (3, 16)), (8, 8, (3, 16)),
+                                    (8, 8, (3, 10))])
 def test_grid_shapes_and_gene_counts(ctx, W, H, gr):
     p, init, target, models = population_case(11 + W, 16, W, H, 150, n_in=3, g_range=gr, l_range=(1, 40))
     ds, got, want = run_both(ctx, p, init, target, models)
@@ -116,7 +117,8 @@ def test_grid_shapes_and_gene_counts(ctx, W, H, gr):
     check_fitness(got["raw_error"], want["raw_error"])
 
 
-@pytest.mark.parametrize("W,H,min_conc", [(32, 32, 0.0), (64, 64, 0.0), (48, 48, 1e-3), (64, 64, 1e-3)])
+@pytest.mark.parametrize("W,H,min_conc", [(32, 32, 0.0), (64, 64, 0.0), (48, 48, 1e-3), (64, 64, 1e-3),
+                                            (8, 8, 1e-3)])
 def test_models_with_non_diffusing_genes(ctx, W, H, min_conc):
     """Models where some genes do not diffuse run the step variant that skips
     their diffusion terms; minConc 0 keeps them on the fast step, a positive
