@@ -392,13 +392,25 @@ def bench_config4(args):
     ctx.close()
 
 
+def coarse_schedule(args):
+    """--coarse-generations as evolve takes it: a count, or (side, count) phases."""
+    v = str(args.coarse_generations)
+    if ":" not in v:
+        return int(v)
+    return [tuple(int(x) for x in ph.split(":")) for ph in v.split(",")]
+
+
 def bench_time_to_fitness(args):
     """Plain batched vs surrogate-screened evolution on the same problems and
     seeds: wall time until the simulator-verified best error reaches each
     threshold.  One seed is one problem (its own blob inputs and hidden-GRN
     target) and one search seed; single runs vary a lot, so the value is the
     median over the seeds of plain time / screened time to the plain run's
-    final best (0 where the screened run never reached it)."""
+    final best (0 where the screened run never reached it).  The plain run
+    has --generations generations; the screened run the same wall time (it
+    ends when it reaches the plain run's final best or when that time is
+    up), so a schedule of cheaper generations is not cut short by their
+    count."""
     import grn
     import evolve
 
@@ -412,13 +424,18 @@ def bench_time_to_fitness(args):
         evolve.make_target(ctx, prob)
         runs = {}
         for name, screened in (("plain", False), ("screened", args.screen != "none")):
-            log(f"[bench] seed {seed}: {name} evolution, {args.generations} generations")
-            runs[name] = evolve.evolve(ctx, prob, screened, max_generations=args.generations, threshold=1e-4,
+            first = name == "plain"
+            log(f"[bench] seed {seed}: {name} evolution, " +
+                (f"{args.generations} generations" if first else f"{runs['plain'].wall[-1]:.1f} s"))
+            runs[name] = evolve.evolve(ctx, prob, screened,
+                                       max_generations=args.generations if first else 20 * args.generations,
+                                       threshold=1e-4 if first else runs["plain"].best[-1],
+                                       time_limit=None if first else runs["plain"].wall[-1],
                                        oversample=args.oversample, train_steps=args.train_steps,
                                        screen_fraction=args.screen_fraction, screen_policy=args.screen_policy,
                                        screen=args.screen if screened else "mlp", coarse_side=args.coarse_side,
                                        pair_weight=args.pair_weight if screened else 0.0,
-                                       coarse_generations=args.coarse_generations if name == "screened" else 0,
+                                       coarse_generations=coarse_schedule(args) if name == "screened" else 0,
                                        progress=log)
         plain, scr = runs["plain"], runs["screened"]
         thresholds = sorted({0.02, 0.01, 0.005, plain.best[-1]}, reverse=True)
@@ -434,7 +451,7 @@ def bench_time_to_fitness(args):
         if args.screen == "mlp":
             sp += [x["spearman"] for x in screened_gens if x["n_candidates"]]
             mae += [x["mae"] for x in screened_gens if x["n_candidates"]]
-            for k in phase:
+            for k in ("predict_ms", "simulate_ms", "record_ms", "breed_ms", "train_ms"):
                 phase[k] += [x[k] for x in screened_gens[2:]]
         elif args.screen == "coarse":
             phase.setdefault("coarse_ms", []).extend(x["coarse_ms"] for x in screened_gens[2:])
@@ -548,9 +565,10 @@ def main():
                          "(none: the second run is plain, with --coarse-generations)")
     ap.add_argument("--coarse-side", type=int, default=8,
                     help="evolve: side of the coarse grid (--screen coarse, --coarse-generations)")
-    ap.add_argument("--coarse-generations", type=int, default=0,
-                    help="evolve: the second run's first generations simulate on the coarse grid only "
-                         "(a coarse-to-fine schedule)")
+    ap.add_argument("--coarse-generations", default="8:100,16:100",
+                    help="evolve: the second run's first generations simulate on coarse grids only "
+                         "(a coarse-to-fine schedule): a count on --coarse-side, or side:count phases; "
+                         "0 for none")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--path", choices=["auto", "lds", "tiled"], default="auto",
                     help="config1: kernel path of the batch (tiled: force the halo-tiled kernels, for comparison)")

@@ -117,8 +117,8 @@ def coarse_dataset(ctx, prob: Problem, side: int):
 def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: float, oversample: int = 8,
            train_steps: int = 20, surrogate_batch: int = 4096, screen_fraction: float = 1.0,
            screen_policy: int = grn.SCREEN_ABSOLUTE, screen: str = "mlp", coarse_side: int = 8,
-           pair_weight: float = 0.0, coarse_generations: int = 0, progress=None, results=None,
-           name: str = "") -> Run:
+           pair_weight: float = 0.0, coarse_generations=0, time_limit: float = None, progress=None,
+           results=None, name: str = "") -> Run:
     """Plain (screened False) or screened evolution.  screen "mlp": the
     learned surrogate scores the candidates; "coarse": a simulation of every
     candidate on a coarse_side x coarse_side grid does.  pair_weight > 0
@@ -128,18 +128,30 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
     child on the coarse_side grid only (Search.run_generation_scaled; their
     best counts as no simulator-verified error, so run.best holds inf), then
     the population is scored at full size (Search.rescore) and the run goes
-    on as configured.  results (optional):
+    on as configured.  A list of (side, generations) phases instead runs
+    the phases in turn, each on its own grid, the population scored again
+    at every change of grid.  time_limit (seconds, optional): the run also
+    ends after the generation that passes it.  results (optional):
     a store.Store that receives the run's parameters, every generation's
     statistics and the final Pareto front."""
     if screened and screen not in ("mlp", "coarse"):
         raise ValueError(f"unknown screen {screen!r}")
-    if coarse_generations < 0 or coarse_generations >= max_generations:
-        raise ValueError("coarse_generations must lie in [0, max_generations)")
+    phases = [(coarse_side, coarse_generations)] if isinstance(coarse_generations, int) else \
+        [(int(a), int(b)) for a, b in coarse_generations]
+    phases = [(side, n) for side, n in phases if n != 0]
+    n_coarse = sum(n for _, n in phases)
+    if any(n < 0 for _, n in phases) or n_coarse >= max_generations:
+        raise ValueError("coarse generations must be >= 0 and fewer than max_generations in all")
     if prob.target is None:
         make_target(ctx, prob)
     ds = ctx.add_dataset(prob.params, prob.init, prob.target)
-    if (screened and screen == "coarse") or coarse_generations:
+    if screened and screen == "coarse":
         ds_c, dif_scale = coarse_dataset(ctx, prob, coarse_side)
+    # per coarse generation: (dataset, diffusion scale, last of its phase)
+    schedule = []
+    for side, n in phases:
+        d, f = coarse_dataset(ctx, prob, side)
+        schedule += [(d, f, i == n - 1) for i in range(n)]
     ctx.set_launch_mode(grn.LAUNCH_GRAPH)
     n_isl = prob.search["n_islands"]
     search = grn.Search(island_datasets=[ds] * n_isl, **prob.search)
@@ -150,7 +162,7 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
                                 seed=prob.search["seed"])
             if pair_weight > 0:
                 sur.set_pair_weight(pair_weight)
-        search.set_oversample(1 if coarse_generations else oversample)
+        search.set_oversample(1 if schedule else oversample)
         search.set_screen_fraction(screen_fraction)
         search.set_screen_policy(screen_policy)
     run = Run(screened)
@@ -159,14 +171,15 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
                                    results.add_search_params(search.params), n_isl)
     t0 = time.perf_counter()
     for gen in range(max_generations):
-        if gen < coarse_generations:
-            g = search.run_generation_scaled(ctx, [ds_c] * n_isl, dif_scale)
+        if gen < n_coarse:
+            d, f, last = schedule[gen]
+            g = search.run_generation_scaled(ctx, [d] * n_isl, f)
             run.cell_steps += ctx.stats()["cell_steps"]
             run.best.append(math.inf)
             run.wall.append(time.perf_counter() - t0)
             run.screen.append({"coarse_best": g["min_fit"]})
-            if gen + 1 == coarse_generations:  # to full size: breed k candidates from the re-scored members
-                if screened:
+            if last:  # the next grid scores the members again
+                if gen + 1 == n_coarse and screened:  # to full size: k candidates per slot from now on
                     search.set_oversample(oversample)
                 search.rescore()
             continue
@@ -187,6 +200,8 @@ def evolve(ctx, prob: Problem, screened: bool, max_generations: int, threshold: 
             progress(f"[evolve] {'screened' if screened else 'plain'} gen {gen} best {g['min_fit']:.4g} "
                      f"t {run.wall[-1]:.1f}s")
         if g["min_fit"] <= threshold or g["done"]:  # target reached, or the search's own end (search.cpp:157-167)
+            break
+        if time_limit is not None and run.wall[-1] >= time_limit:
             break
     # the best model re-simulated by the FP64 kernel (the reference's double
     # precision throughout): the reported error must agree to 1e-5
