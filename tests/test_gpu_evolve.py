@@ -163,9 +163,10 @@ def test_coarse_to_fine_evolution_is_verified_and_deterministic():
     ctx = grn.Context(0)
     prob = evolve.scaled_config4(seed=2, n_islands=4, island_size=128, steps=400)
     evolve.make_target(ctx, prob)
-    for screened, kw in ((False, {}), (True, dict(oversample=4, screen="coarse"))):
+    for screened, kw, phases in ((False, {}, 6), (True, dict(oversample=4, screen="coarse"), 6),
+                                 (False, {}, [(8, 3), (16, 3)])):
         runs = [evolve.evolve(ctx, prob, screened, max_generations=14, threshold=0.0, coarse_side=8,
-                              coarse_generations=6, **kw) for _ in range(2)]
+                              coarse_generations=phases, **kw) for _ in range(2)]
         a, b = runs
         assert a.best == b.best and a.verified and b.verified
         assert a.best[:6] == [float("inf")] * 6 and np.isfinite(a.best[6])
