@@ -36,7 +36,7 @@ class HostPool {
   void run(int n, int max_threads, const std::function<void(int)> &f) {
     const int nt = std::min(n, std::min(max_threads, (int)workers_.size() + 1));
     std::unique_lock<std::mutex> busy(run_mu_, std::defer_lock);
-    if (nt <= 1 || !busy.try_lock()) {
+    if (nt <= 1 || inside() || !busy.try_lock()) {
       for (int i = 0; i < n; ++i) f(i);
       return;
     }
@@ -49,7 +49,9 @@ class HostPool {
       ++epoch_;
     }
     wake_.notify_all();
+    inside() = true;
     drain(f, n);
+    inside() = false;
     std::unique_lock<std::mutex> lk(mu_);
     seats_ = 0;  // a helper that wakes from now on does not join this loop
     idle_.wait(lk, [this] { return running_ == 0; });
@@ -78,11 +80,19 @@ class HostPool {
     for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this] { loop(); });
   }
 
+  // True on a thread that runs a loop's f: a loop started there runs inline
+  // (the pool's own lock is never taken twice by one thread).
+  static bool &inside() {
+    static thread_local bool v = false;
+    return v;
+  }
+
   void drain(const std::function<void(int)> &f, int n) {
     for (int i; (i = next_.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
   }
 
   void loop() {
+    inside() = true;
     unsigned long seen = 0;
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {

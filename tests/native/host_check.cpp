@@ -22,6 +22,7 @@
 #include "grn.h"
 #include "grn_internal.hpp"
 #include "grn_model.hpp"
+#include "grn_pool.hpp"
 #include "grn_program.h"
 #include "grn_search.hpp"
 #include "grn_surrogate.hpp"
@@ -208,6 +209,35 @@ static void searches_replay(int oversample) {
   }
 }
 
+// The worker pool: every index runs exactly once, also when loops start
+// from several threads at once (the later ones run on their own thread) and
+// from inside a loop (nested loops run inline).
+static void pool_loops() {
+  constexpr int kN = 1000;
+  auto one_loop = [](std::vector<int> *hits, bool nested) {
+    grn::parallel_for(kN, [&](int i) {
+      (*hits)[i] += 1;
+      if (nested && i % 100 == 0) {
+        std::vector<int> inner(50, 0);
+        grn::parallel_for(50, [&](int j) { inner[j] += j; });
+        for (int j = 0; j < 50; ++j)
+          if (inner[j] != j) (*hits)[i] = -1000;
+      }
+    });
+  };
+  for (int rep = 0; rep < 20; ++rep) {
+    std::vector<std::vector<int>> hits(4, std::vector<int>(kN, 0));
+    std::vector<std::thread> callers;
+    for (int t = 0; t < 4; ++t) callers.emplace_back(one_loop, &hits[t], t % 2 == 1);
+    for (std::thread &th : callers) th.join();
+    for (const std::vector<int> &h : hits)
+      for (int v : h) check(v == 1, "pool: every index of every loop runs exactly once");
+  }
+  std::vector<int> none;
+  grn::parallel_for(0, [&](int) { none.push_back(1); });
+  check(none.empty(), "pool: an empty loop runs nothing");
+}
+
 int main(int argc, char **argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: host_check <known-answer model file>\n");
@@ -220,6 +250,7 @@ int main(int argc, char **argv) {
   while (!kat.empty() && (kat.back() == '\n' || kat.back() == ' ')) kat.pop_back();
   text_format(kat);
   compile_and_validate();
+  pool_loops();
   searches_replay(1);
   searches_replay(4);
   if (g_failures) {
