@@ -27,9 +27,9 @@ typedef __bf16 bf16;
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16 staged bytes (a native vector: stays in registers)
 
-constexpr int BM = 128, BN = 128, BK = 32;
-constexpr int LDS_STRIDE = 40;  // bf16 per LDS tile row: 32 + 8 padding (80 bytes)
+constexpr int BM = 128, BN = 128, BK = 32;  // BK: the K granule every GEMM needs
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 
@@ -40,15 +40,23 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return cdf + z * pdf;
 }
 
-// C = A . Bt^T for M % TM == 0, N % 128 == 0, K % 32 == 0 (checked by the
+// C = A . Bt^T for M % TM == 0, N % 128 == 0, K % BKT == 0 (checked by the
 // host).  256 threads: 4 waves in a 2 x 2 arrangement, each wave a
-// (TM / 2) x 64 block of 16 x 16 accumulators.  TM = 128, or 64 when M is
-// small enough that 128-row tiles would leave CUs idle.
-template <int EPI, int TM>
+// (TM / 2) x 64 block of 16 x 16 accumulators.  TM = 128, 64 or 32 (with
+// BKT = 128): the launcher picks the tallest tile that still gives each CU
+// two workgroups.  BKT: the K depth
+// of one LDS tile: 128 (64 for 128-row tiles, which would need 68 KB of LDS
+// at 128) where K allows it -- several MFMA k-steps between two barriers
+// and one global-load wait -- else 32.
+template <int EPI, int TM, int BKT>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
-  constexpr int WI = TM / 32;  // 16-row accumulator blocks per wave
-  __shared__ __attribute__((aligned(16))) bf16 As[TM * LDS_STRIDE];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LDS_STRIDE];
+  constexpr int WI = TM / 32;       // 16-row accumulator blocks per wave
+  constexpr int STRIDE = BKT + 8;   // bf16 per LDS tile row: 16 bytes of padding
+  constexpr int VR = BKT / 8;                            // 16-byte vectors per tile row
+  constexpr int NA = TM * VR / 256, NB = BN * VR / 256;  // vectors per thread of A and of Bt
+  static_assert(NA >= 1 && NB >= 1 && VR % NA == 0 && VR % NB == 0, "staging map");
+  __shared__ __attribute__((aligned(16))) bf16 As[TM * STRIDE];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[BN * STRIDE];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * BN;
   const int K = g.K;
@@ -56,16 +64,18 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
   const int ks = (EPI == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
   const int kbeg = blockIdx.z * (K / ks), kend = kbeg + K / ks;
 
-  // each thread stages 16 bf16 (two 16-byte vectors) of one Bt row and
-  // TM / 8 bf16 of one A row (two vectors at TM = 128, one at TM = 64)
-  constexpr int AV = TM / 64;  // A vectors per thread
-  const int lrow = t >> 1, lhalf = t & 1;
-  const int arow = t / (4 / AV), acol = 8 * AV * (t % (4 / AV));
-  const uint4 *pa = (const uint4 *)(g.A + (size_t)(m0 + arow) * K + kbeg + acol);
-  const uint4 *pb = (const uint4 *)(g.Bt + (size_t)(n0 + lrow) * K + kbeg + 16 * lhalf);
-  uint4 ra[AV], rb0 = pb[0], rb1 = pb[1];
+  // thread t stages NA (NB) consecutive 16-byte vectors of one tile row of A
+  // (Bt): VR / NA threads share a row, so one pointer and immediate offsets
+  // serve all of a thread's loads
+  const int arow = t / (VR / NA), acol = 8 * NA * (t % (VR / NA));
+  const int brow = t / (VR / NB), bcol = 8 * NB * (t % (VR / NB));
+  const u32x4 *pa = (const u32x4 *)(g.A + (size_t)(m0 + arow) * K + kbeg + acol);
+  const u32x4 *pb = (const u32x4 *)(g.Bt + (size_t)(n0 + brow) * K + kbeg + bcol);
+  u32x4 ra[NA], rb[NB];
 #pragma unroll
-  for (int v = 0; v < AV; ++v) ra[v] = pa[v];
+  for (int q = 0; q < NA; ++q) ra[q] = pa[q];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) rb[q] = pb[q];
 
   f32x4 acc[WI][4];
 #pragma unroll
@@ -73,34 +83,37 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(MlpGemm g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  uint4 *sa = (uint4 *)(As + arow * LDS_STRIDE + acol);
-  uint4 *sb = (uint4 *)(Bs + lrow * LDS_STRIDE + 16 * lhalf);
   const int frow = lane & 15, fk = 8 * (lane >> 4);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+  for (int k0 = kbeg; k0 < kend; k0 += BKT) {
     __syncthreads();  // the previous tile's fragment reads are done
 #pragma unroll
-    for (int v = 0; v < AV; ++v) sa[v] = ra[v];
-    sb[0] = rb0;
-    sb[1] = rb1;
+    for (int q = 0; q < NA; ++q) ((u32x4 *)(As + arow * STRIDE + acol))[q] = ra[q];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) ((u32x4 *)(Bs + brow * STRIDE + bcol))[q] = rb[q];
     __syncthreads();
-    if (k0 + BK < kend) {  // prefetch the next K slice while this one multiplies
-      pa += 4;
-      pb += 4;
+    if (k0 + BKT < kend) {  // prefetch the next K slice while this one multiplies
+      pa += VR;
+      pb += VR;
 #pragma unroll
-      for (int v = 0; v < AV; ++v) ra[v] = pa[v];
-      rb0 = pb[0];
-      rb1 = pb[1];
+      for (int q = 0; q < NA; ++q) ra[q] = pa[q];
+#pragma unroll
+      for (int q = 0; q < NB; ++q) rb[q] = pb[q];
     }
-    bf16x8 af[WI], bfr[4];
 #pragma unroll
-    for (int i = 0; i < WI; ++i)
-      af[i] = *(const bf16x8 *)(As + ((TM / 2) * wm + 16 * i + frow) * LDS_STRIDE + fk);
+    for (int kk = 0; kk < BKT; kk += 32) {
+      bf16x8 af[WI], bfr[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8 *)(Bs + (64 * wn + 16 * j + frow) * LDS_STRIDE + fk);
+      for (int i = 0; i < WI; ++i)
+        af[i] = *(const bf16x8 *)(As + ((TM / 2) * wm + 16 * i + frow) * STRIDE + kk + fk);
 #pragma unroll
-    for (int i = 0; i < WI; ++i)
+      for (int j = 0; j < 4; ++j)
+        bfr[j] = *(const bf16x8 *)(Bs + (64 * wn + 16 * j + frow) * STRIDE + kk + fk);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < WI; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
   }
 
   const int M = g.M, N = g.N;
@@ -277,28 +290,38 @@ __global__ __launch_bounds__(256) void gather_kernel(const bf16 *__restrict__ fe
 
 using namespace grn_mlp;
 
+template <int TM, int BKT>
+static void launch_gemm(int epi, const MlpGemm &g, dim3 grid, hipStream_t s) {
+  const dim3 block(256);
+  if (epi == EPI_FWD)
+    hipLaunchKernelGGL((gemm_nt_kernel<EPI_FWD, TM, BKT>), grid, block, 0, s, g);
+  else if (epi == EPI_BWD)
+    hipLaunchKernelGGL((gemm_nt_kernel<EPI_BWD, TM, BKT>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, TM, BKT>), grid, block, 0, s, g);
+}
+
 hipError_t grn_mlp_gemm(int epi, const MlpGemm &g, hipStream_t s) {
   const int ks = (epi == EPI_F32 && g.ksplit > 1) ? g.ksplit : 1;
   if (ks > 1 && (!g.Cpart || g.K % (BK * ks))) return hipErrorInvalidValue;
-  // 64-row tiles when 128-row ones would give fewer workgroups than the
-  // device has CUs (e.g. 4096 x 512 outputs: 128 -> 256 workgroups)
-  const bool small = (long)(g.N / BN) * (g.M / BM) * ks < 256 && g.M % 64 == 0;
-  const dim3 grid(g.N / BN, g.M / (small ? 64 : BM), ks), block(256);
-  if (small) {
-    if (epi == EPI_FWD)
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_FWD, 64>), grid, block, 0, s, g);
-    else if (epi == EPI_BWD)
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_BWD, 64>), grid, block, 0, s, g);
-    else
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, 64>), grid, block, 0, s, g);
-  } else {
-    if (epi == EPI_FWD)
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_FWD, BM>), grid, block, 0, s, g);
-    else if (epi == EPI_BWD)
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_BWD, BM>), grid, block, 0, s, g);
-    else
-      hipLaunchKernelGGL((gemm_nt_kernel<EPI_F32, BM>), grid, block, 0, s, g);
-  }
+  // deep K tiles where every workgroup's K slice is a multiple of 128
+  const bool deep = (g.K / ks) % 128 == 0;
+  // the tallest row tile that still gives at least two workgroups per CU
+  // (512; one per CU leaves each global-load wait unhidden), down to 64 rows
+  // (32 with deep tiles); a K-split GEMM keeps 128-row tiles (measured: its
+  // 16 slices already give each CU a workgroup, and 64-row tiles were 10 %
+  // slower)
+  auto tiles = [&](int tm) { return (long)(g.N / BN) * (g.M / tm) * ks; };
+  int tm = BM;
+  if (ks == 1 && tiles(BM) < 512 && g.M % 64 == 0) tm = 64;
+  if (tm == 64 && deep && tiles(64) < 512 && g.M % 32 == 0) tm = 32;
+  const dim3 grid(g.N / BN, g.M / tm, ks);
+  if (tm == 32)
+    launch_gemm<32, 128>(epi, g, grid, s);
+  else if (tm == 64)
+    deep ? launch_gemm<64, 128>(epi, g, grid, s) : launch_gemm<64, BK>(epi, g, grid, s);
+  else
+    deep ? launch_gemm<BM, 64>(epi, g, grid, s) : launch_gemm<BM, BK>(epi, g, grid, s);
   if (ks > 1) {
     const long n4 = (long)g.M * g.N / 4;
     hipLaunchKernelGGL(ksum_kernel, dim3((unsigned)std::min<long>((n4 + 255) / 256, 1024)), dim3(256), 0, s,
