@@ -140,6 +140,34 @@ def test_models_with_non_diffusing_genes(ctx, W, H, min_conc):
         check_fields(f, fo)
 
 
+@pytest.mark.parametrize("W,H,path", [(32, 32, grn.PATH_AUTO), (96, 96, grn.PATH_TILED)])
+def test_links_repeating_a_regulator(ctx, W, H, path):
+    """Models with several links between the same pair of genes (the
+    generators draw distinct pairs; the reference's model format allows
+    repeats, and each link is its own Hill term in the fold).  Fitness and
+    fields agree with the oracle, on chip and halo-tiled."""
+    p, init, target, models = population_case(91 + W, 12, W, H, 200, n_in=2, g_range=(3, 8), l_range=(3, 20))
+    rng = np.random.default_rng(W)
+    repeated = []
+    for m in models:
+        extra = m.links[rng.choice(len(m.links), size=min(4, len(m.links)), replace=False)].copy()
+        extra["dis_const"] = rng.uniform(0.05, 1.0, len(extra))
+        hill = rng.uniform(1.0, 4.0, len(extra))
+        act = rng.random(len(extra)) < 0.5
+        extra["hill_coef"] = np.where(act, hill, -hill)
+        extra["is_and"] = np.where(act, rng.random(len(extra)) < 0.5, True).astype(np.int32)
+        repeated.append(grn.Model(m.genes, np.concatenate([m.links, extra])))
+    print(f"[gpu] repeated-regulator links on {W}x{H}", flush=True)
+    ds = ctx.add_dataset(p, init, target)
+    got = ctx.evaluate_batch(repeated, datasets=ds, path=path)
+    want = ora.evaluate_batch(ora.from_simparams(p), init, target, repeated, n_threads=8)
+    check_fitness(got["raw_error"], want["raw_error"])
+    for m in repeated[:2]:
+        f = ctx.simulate_fields(m, dataset=ds)
+        _, fo = ora.evaluate(ora.from_simparams(p), init, target, m.genes, m.links, want_fields=True)
+        check_fields(f, fo)
+
+
 def check_exact_run(p, got, want):
     """Every model: the same step count, the raw error and the rounded
     fitness within 1e-5 of the oracle, the same ranking."""
