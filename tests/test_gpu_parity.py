@@ -109,7 +109,7 @@ def test_plain_equals_batched_bitwise(ctx):
 
 @pytest.mark.parametrize("W,H,gr", [(32, 32, (3, 16)), (48, 48, (3, 12)), (64, 64, (4, 9)),
                                     (16, 16, (3, 8)), (20, 12, (3, 8)), (7, 5, (3, 16)), (8, 8, (3, 16)),
-                                    (8, 8, (3, 10))])
+                                    (8, 8, (3, 10)), (16, 16, (3, 10))])
 def test_grid_shapes_and_gene_counts(ctx, W, H, gr):
     p, init, target, models = population_case(11 + W, 16, W, H, 150, n_in=3, g_range=gr, l_range=(1, 40))
     ds, got, want = run_both(ctx, p, init, target, models)
