@@ -157,6 +157,24 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in r.stderr
 
 
+def test_coarse_schedule_arguments():
+    """--coarse-generations: a count on --coarse-side or side:count phases;
+    evolve() refuses negative phases and schedules that leave no full-size
+    generation (checked before any device work)."""
+    import argparse
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    import evolve
+    assert bench.coarse_schedule(argparse.Namespace(coarse_generations="100")) == 100
+    assert bench.coarse_schedule(argparse.Namespace(coarse_generations="8:100,16:50")) == [(8, 100), (16, 50)]
+    prob = evolve.scaled_config4(seed=0, n_islands=1, island_size=8)
+    for bad in (-1, 5, [(8, 3), (16, 2)], [(8, -1)]):
+        with pytest.raises(ValueError):
+            evolve.evolve(None, prob, False, max_generations=5, threshold=0.0, coarse_generations=bad)
+
+
 def test_decode_rgb_matches_oracle_and_known_pixels():
     """grn_decode_rgb (the library's SimState::loadMorphologyImage) against
     the oracle's restatement and against pixels with known channel bytes."""
