@@ -191,32 +191,62 @@ __global__ __launch_bounds__(256) void out_forward_kernel(const bf16 *__restrict
 // the loss adds pair_w * ((y_2i - y_2i+1) - (t_2i - t_2i+1))^2 per pair, so
 // dy[b] gains 2 pair_w e_b / n_valid with e_b the pair's difference error
 // seen from row b (the partner's is -e_b).
+// Rows per workgroup of the kernels that also write a transposed copy: the
+// workgroup's rows pass through an LDS tile [column][row], so each column of
+// the copy is written as GR consecutive values (64 bytes) instead of one
+// 2-byte store per value GR columns apart.
+constexpr int GR = 32;
+typedef bf16 TransposeTile[MLP_WIDTH][GR + 2];  // +2: the rows of two columns start in different banks
+
+// Writes rows r0 .. r0 + GR - 1 (those below `rows`) of the tile into the
+// transposed matrix Xt[MLP_WIDTH][rows]: four rows (8 bytes) per store where
+// they are aligned, single values otherwise.
+__device__ __forceinline__ void store_transposed(const TransposeTile &tile, int r0, int rows, bf16 *__restrict__ Xt) {
+  const int nr = rows - r0 < GR ? rows - r0 : GR;
+  const bool vec = (rows & 3) == 0;
+  for (int q = threadIdx.x; q < MLP_WIDTH * (GR / 4); q += blockDim.x) {
+    const int c = q / (GR / 4), r = 4 * (q % (GR / 4));
+    bf16 *dst = Xt + (size_t)c * rows + r0 + r;
+    if (vec && r + 4 <= nr) {
+      const bf16x4 v = {tile[c][r], tile[c][r + 1], tile[c][r + 2], tile[c][r + 3]};
+      *(bf16x4 *)dst = v;
+    } else {
+      for (int e = 0; e < 4 && r + e < nr; ++e) dst[e] = tile[c][r + e];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void out_backward_kernel(const float *__restrict__ y, const float *__restrict__ tgt,
                                                            const float *__restrict__ mask, const bf16 *__restrict__ w,
                                                            const float *__restrict__ Z, float inv_n, float pair_w,
                                                            int rows, float *__restrict__ dy, float *__restrict__ sq,
                                                            bf16 *__restrict__ dZ, bf16 *__restrict__ dZt) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  const float e = mask[row] * (y[row] - tgt[row]);
-  float ep = 0.f;
-  if (pair_w > 0.f) {
-    const int q = row ^ 1;
-    ep = mask[row] * mask[q] * ((y[row] - y[q]) - (tgt[row] - tgt[q]));
-  }
-  const float d = 2.f * (e + pair_w * ep) * inv_n;
-  if (lane == 0) {
-    dy[row] = d;
-    sq[row] = e * e + 0.5f * pair_w * ep * ep;  // each pair's term is shared by its two rows
-  }
+  __shared__ TransposeTile tile;
+  const int r0 = blockIdx.x * GR, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bf16x8 ww = *(const bf16x8 *)(w + 8 * lane);
+  for (int i = wave; i < GR && r0 + i < rows; i += 4) {  // one wave per row
+    const int row = r0 + i;
+    const float e = mask[row] * (y[row] - tgt[row]);
+    float ep = 0.f;
+    if (pair_w > 0.f) {
+      const int q = row ^ 1;
+      ep = mask[row] * mask[q] * ((y[row] - y[q]) - (tgt[row] - tgt[q]));
+    }
+    const float d = 2.f * (e + pair_w * ep) * inv_n;
+    if (lane == 0) {
+      dy[row] = d;
+      sq[row] = e * e + 0.5f * pair_w * ep * ep;  // each pair's term is shared by its two rows
+    }
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int col = 8 * lane + k;
-    const float v = d * (float)ww[k] * gelu_grad(Z[(size_t)row * MLP_WIDTH + col]);
-    dZ[(size_t)row * MLP_WIDTH + col] = (bf16)v;
-    dZt[(size_t)col * rows + row] = (bf16)v;
+    for (int k = 0; k < 8; ++k) {
+      const int col = 8 * lane + k;
+      const bf16 v = (bf16)(d * (float)ww[k] * gelu_grad(Z[(size_t)row * MLP_WIDTH + col]));
+      dZ[(size_t)row * MLP_WIDTH + col] = v;
+      tile[col][i] = v;
+    }
   }
+  __syncthreads();
+  store_transposed(tile, r0, rows, dZt);
 }
 
 // out[r] = sum_c X[r][c] * (v ? v[c] : 1) over a bf16 matrix with `cols`
@@ -276,14 +306,19 @@ __global__ __launch_bounds__(256) void gather_kernel(const bf16 *__restrict__ fe
                                                      const int *__restrict__ idx, int rows, float mu, float inv_sigma,
                                                      bf16 *__restrict__ X, bf16 *__restrict__ Xt,
                                                      float *__restrict__ t) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  const int src = idx[row];
-  const bf16x8 x = *(const bf16x8 *)(feat + (size_t)src * MLP_WIDTH + 8 * lane);
-  *(bf16x8 *)(X + (size_t)row * MLP_WIDTH + 8 * lane) = x;
+  __shared__ TransposeTile tile;
+  const int r0 = blockIdx.x * GR, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < GR && r0 + i < rows; i += 4) {  // one wave per row
+    const int row = r0 + i;
+    const int src = idx[row];
+    const bf16x8 x = *(const bf16x8 *)(feat + (size_t)src * MLP_WIDTH + 8 * lane);
+    *(bf16x8 *)(X + (size_t)row * MLP_WIDTH + 8 * lane) = x;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) Xt[(size_t)(8 * lane + k) * rows + row] = x[k];
-  if (lane == 0) t[row] = (target[src] - mu) * inv_sigma;
+    for (int k = 0; k < 8; ++k) tile[8 * lane + k][i] = x[k];
+    if (lane == 0) t[row] = (target[src] - mu) * inv_sigma;
+  }
+  __syncthreads();
+  store_transposed(tile, r0, rows, Xt);
 }
 
 }  // namespace grn_mlp
@@ -339,7 +374,7 @@ hipError_t grn_mlp_out_backward(const float *y, const float *t, const float *mas
                                 float inv_n, float pair_w, int rows, float *dy, float *sq, __bf16 *dZ, __bf16 *dZt,
                                 hipStream_t s) {
   if (pair_w > 0.f && rows % 2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(out_backward_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, y, t, mask, w, Z, inv_n, pair_w,
+  hipLaunchKernelGGL(out_backward_kernel, dim3((rows + GR - 1) / GR), dim3(256), 0, s, y, t, mask, w, Z, inv_n, pair_w,
                      rows, dy, sq, dZ, dZt);
   return hipGetLastError();
 }
@@ -361,7 +396,7 @@ hipError_t grn_mlp_adam(const MlpAdam &a, hipStream_t s) {
 
 hipError_t grn_mlp_gather(const __bf16 *feat, const float *target, const int *idx, int rows, float mu,
                           float inv_sigma, __bf16 *X, __bf16 *Xt, float *t, hipStream_t s) {
-  hipLaunchKernelGGL(gather_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, feat, target, idx, rows, mu, inv_sigma, X,
+  hipLaunchKernelGGL(gather_kernel, dim3((rows + GR - 1) / GR), dim3(256), 0, s, feat, target, idx, rows, mu, inv_sigma, X,
                      Xt, t);
   return hipGetLastError();
 }
